@@ -1,0 +1,166 @@
+"""EAM step throughput on one MI355X: atom-steps/s of the 16M-atom FeCuNi box.
+
+Workload: BCC FeCuNi, 200^3 cells (16,000,000 atoms), Fe:Cu:Ni = 985:10:5
+placed by the reference's world builder, 300 K, dt = 1 fs, synthetic
+3-element setfl table (Nr = Nrho = 5000, cutoff 5.6 A).  A step is one
+complete velocity-Verlet step of the resident state (rho/F' pass, force +
+Verlet pass, ownership decisions, periodic halo), state resident in HBM.
+
+Prints one JSON line: the FP32-mixed rate is `value`; the plain FP64 rate,
+the mixed-vs-FP64 force error, the roofline of the dominant kernel (from HIP
+events on the library's stream) and the single-core CPU oracle rate on a
+bounded sample of the same workload sit beside it.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "misa-md_amd"))
+
+import misa_eam as M  # noqa: E402
+import setfl_gen  # noqa: E402
+
+METRIC = "atom-steps/s, 16M-atom FeCuNi EAM: FP32-mixed vs FP64 at ≤1e-5 force err"
+HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per atom per launch of the force + Verlet pass:
+# read x (24), type (1), F' (8 | 4), v (24); write x (24), v (24)
+FORCE_BYTES = {"fp64": 105, "mixed": 101}
+RHO_BYTES = {"fp64": 33, "mixed": 29}  # read x, type; write F'
+STEP_BYTES = {"fp64": 138, "mixed": 130}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_precision(dev, mode, steps, warmup):
+    dev.set_precision(mode)
+    dev.compute_forces()
+    if warmup:
+        dev.step(warmup)
+    t0 = time.perf_counter()
+    dev.step(steps)
+    secs = time.perf_counter() - t0
+    # per-kernel device time with HIP events on the library's stream
+    dev.set_profiling(True)
+    dev.step(max(5, min(steps, 20)))
+    tim = dev.last_timings()
+    dev.set_profiling(False)
+    return secs, tim
+
+
+def force_error(dev):
+    atoms, _ = dev.download()
+    dev.set_precision(M.PREC_FP64)
+    dev.compute_forces()
+    a64, _ = dev.download()
+    dev.set_precision(M.PREC_MIXED)
+    dev.compute_forces()
+    a32, _ = dev.download()
+    sel = a64["type"] >= 0
+    g = dev.ghost
+    sb = M.sub_box_view(np.arange(len(a64)), dev).reshape(-1)
+    f64 = a64["f"][sb]
+    f32 = a32["f"][sb]
+    rms = np.sqrt(np.mean(np.sum(f64 * f64, axis=1)))
+    norm = np.maximum(np.linalg.norm(f64, axis=1), rms)
+    err = np.linalg.norm(f32 - f64, axis=1) / norm
+    del atoms, sel, g
+    return float(err.max()), float(np.sqrt(np.mean(err * err))), float(rms)
+
+
+def cpu_baseline(setfl):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    exe = oracle_py.BASELINE_BIN
+    if not os.path.exists(exe):
+        return None
+    cells, steps = 40, 10
+    out = subprocess.run([exe, setfl, str(cells), str(cells), str(cells), "985", "10", "5", "300", "2020",
+                          str(steps)], check=True, capture_output=True, text=True, timeout=600)
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return {"value": r["atom_steps_per_s"], "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": "oracle (reference single-rank step restated: half lists, 104-B AoS, reference phase order) "
+                      "on FeCuNi 985:10:5 BCC %d^3 cells (%d atoms), 300 K, %d steps, one thread; %s" % (
+                          cells, 2 * cells ** 3, steps, "phase seconds " + json.dumps(r["phase_s"]))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cells", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs the EAM step on one GPU only; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    n = args.cells
+    natoms = 2 * n ** 3
+    with tempfile.TemporaryDirectory() as tmp:
+        pot = os.path.join(tmp, "fecuni.eam.alloy")
+        setfl_gen.write_setfl(pot, ("Fe", "Cu", "Ni"))
+        log("building %d^3 cells (%d atoms)" % (n, natoms))
+        dev = M.EamDevice((n, n, n))
+        dev.make_offsets()
+        dev.set_potential(pot)
+        dev.build_world(2020, 300.0, (985, 10, 5), 1)
+        dev.set_timestep(0.001)
+        log("fp64 run")
+        s64, t64 = run_precision(dev, M.PREC_FP64, args.steps, args.warmup)
+        log("mixed run")
+        s32, t32 = run_precision(dev, M.PREC_MIXED, args.steps, args.warmup)
+        log("force error")
+        fmax, frms, f_scale = force_error(dev)
+        th = dev.thermo()
+        dev.close()
+        cpu = None if args.no_cpu else cpu_baseline(pot)
+
+    rate32 = natoms * args.steps / s32
+    rate64 = natoms * args.steps / s64
+    avg_force_ms = t32["force_ms"] / max(1, t32["launches"][1])
+    avg_rho_ms = t32["rho_df_ms"] / max(1, t32["launches"][0])
+    achieved = natoms * FORCE_BYTES["mixed"] / (avg_force_ms * 1e-3) / 1e9
+    avg_force64 = t64["force_ms"] / max(1, t64["launches"][1])
+    res = {
+        "metric": METRIC,
+        "value": rate32,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": s32 * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-mixed (f32 splines/pair terms, f64 positions/Verlet/sums)",
+        "data": "synthetic",
+        "config": {"workload": "BCC FeCuNi %d^3 cells (%d atoms), Fe:Cu:Ni 985:10:5, 300 K, dt 1 fs, "
+                               "synthetic 3-element setfl Nr=Nrho=5000 rc 5.6 A" % (n, natoms),
+                   "atoms": natoms, "parallelism": "single GPU"},
+        "fp64": {"value": rate64, "unit": "atom-steps/s", "ms_per_step": s64 * 1e3 / args.steps,
+                 "force_kernel_ms": avg_force64, "rho_kernel_ms": t64["rho_df_ms"] / max(1, t64["launches"][0])},
+        "mixed_over_fp64": rate32 / rate64,
+        "force_err_mixed_vs_fp64": {"max": fmax, "rms": frms, "rms_force_eV_per_A": f_scale},
+        "roofline": {"bound": "hbm", "kernel": "kForce (force + Verlet pass, mixed)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_atom": FORCE_BYTES["mixed"],
+                     "avg_launch_ms": avg_force_ms, "rho_kernel_ms": avg_rho_ms},
+        "step_roofline_frac": rate32 * STEP_BYTES["mixed"] / (HBM_PEAK_GBS * 1e9),
+        "thermo": {"temperature": th["temperature"], "e_potential_per_atom": th["e_potential"] / natoms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

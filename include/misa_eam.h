@@ -1,0 +1,188 @@
+/*
+ * misa_eam.h - C-ABI of the MI355X EAM step library (libmisa_eam_hip.so).
+ *
+ * The reference (MISA-MD v0.4.0) reaches an accelerator through eight C++
+ * hooks named ${ARCH_NAME}_* (src/arch/arch_imp.h:17-31), selected at build
+ * time by MD_HIP_ARCH_ENABLE_FLAG (config.cmake:17, arch/arch_configure.cmake
+ * :13-16) and linked into libmd as target md_arch_hip
+ * (arch/arch_libs.cmake:17-38).  Their arguments are C++ types from libcomm
+ * (comm::BccDomain), pot (eam) and MISA-MD itself (NeighbourIndex,
+ * AtomElement).  This header flattens them into plain C: every argument is a
+ * pointer plus sizes, every call returns 0 on success or a negative MISA_E_*
+ * code, and misa_last_error() gives the message.  A thin C++ adapter that
+ * defines the eight hip_* symbols on top of these calls makes the library a
+ * drop-in md_arch_hip at one rank.
+ *
+ * Two modes share one handle:
+ *  - hook mode (misa_lat_rho / misa_lat_df / misa_lat_force) works on the
+ *    reference's own host array of 104-byte AtomElement slots, synchronously,
+ *    exactly where atom::latRho/latDf/latForce call the hooks;
+ *  - resident mode (misa_upload_state ... misa_step ... misa_download_state)
+ *    keeps the whole state in HBM and runs complete velocity-Verlet steps
+ *    (lattice and inter atoms, ownership decisions, periodic halo) on the
+ *    GPU; it replaces the loop body of simulation::simulate
+ *    (simulation.cpp:164-194), for which the reference has no hook.
+ *
+ * Threading: one handle per host thread; calls on a handle are serialised
+ * and run on that handle's HIP stream.  Every call returns after its work
+ * has finished on the device.
+ */
+#ifndef MISA_EAM_H
+#define MISA_EAM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct misa_handle misa_handle;
+
+/* error codes */
+#define MISA_OK 0
+#define MISA_E_ARG -1        /* bad argument or call order */
+#define MISA_E_HIP -2        /* HIP runtime error */
+#define MISA_E_IO -3         /* potential file missing or malformed */
+#define MISA_E_DEVICE -4     /* a kernel raised a device-side error flag */
+#define MISA_E_CAPACITY -5   /* a fixed-capacity device list overflowed */
+
+/* precision of the resident-mode force path */
+#define MISA_PREC_FP64 0     /* FP64 everywhere */
+#define MISA_PREC_MIXED 1    /* FP32 splines and pair terms, FP64 positions/Verlet/sums */
+
+/* Geometry of the single-rank domain: the whole periodic box.  Mirrors the
+ * comm::BccDomain fields the reference reads (simulation.cpp:29-56):
+ * phase_space (cells per dimension), lattice_const, cutoff_radius_factor;
+ * the ghost width is ceil(cutoff_radius_factor) + 1 cells as the reference
+ * sets it (simulation.cpp:46). */
+typedef struct misa_geom {
+  int64_t phase_space[3];
+  double lattice_const;
+  double cutoff_radius_factor;
+} misa_geom;
+
+/* Thermodynamic reductions over the resident state (FP64 on the device). */
+typedef struct misa_thermo {
+  double mvv;            /* sum m v^2 over lattice and inter atoms (system_configuration.cpp:61-84) */
+  double kinetic;        /* mvv / 2 (reduceEnergy, system_configuration.cpp:113-126) */
+  double temperature;    /* mvv * mvv2e / ((3N-3) k_B) (system_configuration.cpp:51-59) */
+  double e_embed;        /* sum_i F_i(rho_i) */
+  double e_pair;         /* 1/2 sum_{i != j} phi_ij(r_ij) */
+  double e_potential;    /* e_embed + e_pair */
+  double virial;         /* sum over pairs of r_ij . f_ij */
+  double force_sum[3];   /* sum_i f_i (system_configuration.cpp:8-24) */
+  double max_speed;      /* max_i |v_i| */
+  int64_t n_lattice;     /* occupied lattice sites */
+  int64_t n_inter;       /* inter atoms in the local list */
+  int64_t n_vacancy;     /* vacant lattice sites */
+} misa_thermo;
+
+/* ---- set-up: hip_env_init / hip_domain_init (arch_env.hpp:15-19,
+ *      simulation.cpp:52-54) ---- */
+
+/* Create a handle on HIP device `device` for the domain `g`. */
+int misa_create(const misa_geom* g, int device, misa_handle** out);
+/* hip_env_clean (arch_env.hpp:21-25): free every device buffer of `h`. */
+int misa_destroy(misa_handle* h);
+/* Message of the last failing call on `h` (never NULL). */
+const char* misa_last_error(const misa_handle* h);
+
+/* hip_nei_offset_init (simulation.cpp:67-69, neighbour_index.h:22-25): the
+ * four linear-offset lists NeighbourIndex::make builds in the ghost-extended
+ * doubled-x array (neighbour_index.inl:13-76): full lists for even and odd x
+ * and the lexicographically positive halves. */
+int misa_set_offsets(misa_handle* h, const int64_t* even, int64_t n_even, const int64_t* odd, int64_t n_odd,
+                     const int64_t* half_even, int64_t n_half_even, const int64_t* half_odd, int64_t n_half_odd);
+/* Build the same four lists on the host, as NeighbourIndex::make does. */
+int misa_make_offsets(misa_handle* h);
+/* Copy list `which` (0 even, 1 odd, 2 half even, 3 half odd) into `out`;
+ * returns its length through `n`. */
+int misa_get_offsets(const misa_handle* h, int which, int64_t* out, int64_t cap, int64_t* n);
+
+/* hip_pot_init (simulation.cpp:133): parse a DYNAMO setfl (.eam.alloy) file
+ * and build the interpolation tables the pot library builds in
+ * eam::interpolateFile (simulation.cpp:131). */
+int misa_set_potential_file(misa_handle* h, const char* setfl_path);
+/* Copy the 7-coefficient table (per knot: derivative cubic c0..c2, value
+ * cubic c3..c6) of kind 0 = F(rho), 1 = rho(r), 2 = r*phi(r) for file element
+ * index `idx` (pair index i*(i+1)/2+j for kind 2).  Returns its length. */
+int misa_get_potential_table(const misa_handle* h, int kind, int idx, double* out, int64_t cap, int64_t* n);
+
+/* ---- hook mode: hip_eam_rho_calc / hip_eam_df_calc / hip_eam_force_calc
+ *      (atom.cpp:159-160, 293-294, 320-321) ----
+ * `atoms` is the host AtomList array (104-byte AtomElement slots, ghost
+ * extended, atom_list.cpp:18).  The caller has zeroed rho and f
+ * (atom::clearForce).  On return, rho (resp. df, f) of every non-vacant
+ * sub-box slot holds the complete lattice-lattice sum; ghost slots keep
+ * zero partials, so the reference's reverse halos add nothing. */
+int misa_lat_rho(misa_handle* h, void* atoms, double cutoff_radius);
+int misa_lat_df(misa_handle* h, void* atoms, double cutoff_radius);
+int misa_lat_force(misa_handle* h, void* atoms, double cutoff_radius);
+
+/* ---- resident mode ---- */
+
+/* Upload the whole state: the ghost-extended AtomElement array (only the
+ * sub-box slots are read) and the inter-atom list (n_inter AtomElement
+ * records in list order). */
+int misa_upload_state(misa_handle* h, const void* atoms, const void* inter, int64_t n_inter);
+/* Download into the same layout; ghost slots are filled from their periodic
+ * images as exchangeAtom leaves them.  `n_inter` returns the inter count;
+ * MISA_E_CAPACITY if it exceeds `inter_cap`. */
+int misa_download_state(misa_handle* h, void* atoms, void* inter, int64_t inter_cap, int64_t* n_inter);
+/* Build the initial state the way WorldBuilder::build does
+ * (world_builder.cpp:64-103) and upload it: BCC sites, velocities from
+ * std::mt19937(seed), alloy types from glibc rand() after srand(alloy_seed)
+ * (alloy_seed = 1 reproduces the reference's unseeded rand()), zero
+ * momentum, rescale to tset when tset != 0. */
+int misa_build_world(misa_handle* h, uint32_t seed, double tset, const int32_t ratio[3], uint32_t alloy_seed);
+
+int misa_set_precision(misa_handle* h, int mode);
+/* NewtonMotion::setTimestepLength (newton_motion.cpp:13-17). */
+int misa_set_timestep(misa_handle* h, double dt);
+/* Inter-atom list capacity (default 65536 entries). */
+int misa_set_inter_capacity(misa_handle* h, int64_t cap);
+
+/* clearForce + computeEam on the resident state (simulation.cpp:141-143):
+ * the first force evaluation before the step loop. */
+int misa_compute_forces(misa_handle* h);
+/* `nsteps` iterations of simulation::simulate's loop body
+ * (simulation.cpp:164-194): half kick and drift, decide, inter-atom
+ * migration and periodic halo, EAM forces, half kick. */
+int misa_step(misa_handle* h, int64_t nsteps);
+/* atom::setv (atom.cpp:475-494): add the PKA velocity to the atom at lattice
+ * coordinate (lat[0], lat[1], lat[2], basis lat[3]). */
+int misa_setv(misa_handle* h, const int32_t lat[4], const double dir[3], double energy);
+/* simulation::collisionStep (simulation.cpp:208-217): setv, halo, forces. */
+int misa_collision_step(misa_handle* h, const int32_t lat[4], const double dir[3], double energy);
+/* configuration::rescale (system_configuration.cpp:86-111). */
+int misa_rescale(misa_handle* h, double tset);
+/* Reductions over the current state (forces from the last evaluation). */
+int misa_thermo_compute(misa_handle* h, misa_thermo* out);
+
+/* Device time (ms, from HIP events on the handle's stream) of the most
+ * recent misa_step call, split by kernel class: [0] rho/df pass, [1] force +
+ * Verlet pass, [2] inter-atom and ownership work, [3] whole call. */
+int misa_last_timings(const misa_handle* h, double out[4]);
+/* Record HIP events around every kernel launch of misa_step (on = 1) so
+ * misa_last_timings splits the device time by kernel class;
+ * misa_last_launch_counts gives the number of launches per class. */
+int misa_set_profiling(misa_handle* h, int on);
+int misa_last_launch_counts(const misa_handle* h, int64_t out[3]);
+
+/* ---- host-only set-up: no device access, usable without a GPU ---- */
+
+/* NeighbourIndex::make for geometry `g` (which = 0..3 as misa_get_offsets;
+ * 4, 5 = pruned parity-plane slot deltas of the even / odd centres). */
+int misa_host_offsets(const misa_geom* g, int which, int64_t* out, int64_t cap, int64_t* n);
+/* Interpolation table (as misa_get_potential_table) straight from a file. */
+int misa_host_potential_table(const char* setfl_path, int kind, int idx, double* out, int64_t cap, int64_t* n);
+/* WorldBuilder::build into a caller-provided ghost-extended AtomElement
+ * array of (2(n0+2g)) (n1+2g) (n2+2g) slots. */
+int misa_host_build_world(const misa_geom* g, uint32_t seed, double tset, const int32_t ratio[3], uint32_t alloy_seed,
+                          void* atoms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MISA_EAM_H */

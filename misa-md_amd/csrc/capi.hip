@@ -1,0 +1,703 @@
+// C-ABI of the MI355X EAM library (declared in include/misa_eam.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/misa_eam.h"
+#include "device_common.hpp"
+#include "hook.hpp"
+#include "host_setup.hpp"
+#include "resident.hpp"
+
+using namespace misa;
+
+struct misa_handle {
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Box box{};
+  DevState s{};
+  Offsets ref{};
+  bool have_offsets = false;
+  LatOffsets lat{};
+  FullOffsets full{};
+  Setfl setfl{};
+  bool have_pot = false;
+  std::vector<std::vector<double>> cF, cR, cZ;  // 7-coefficient tables by file element
+  bool species_present[3] = {false, false, false};
+  double* tab64 = nullptr;
+  float* tab32 = nullptr;
+  Tables<double> t64{};
+  Tables<float> t32{};
+  int precision = MISA_PREC_FP64;
+  int acc = 0;
+  double dt = 0.001;
+  bool state_ready = false;
+  bool forces_ready = false;
+  AtomElement* aos_dev = nullptr;
+  std::vector<void*> allocs;
+  bool profile = false;
+  std::vector<hipEvent_t> events;
+  double timings[4] = {0, 0, 0, 0};
+  int64_t launches[3] = {0, 0, 0};
+};
+
+namespace {
+
+int fail(misa_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIPCHECK(h, call)                                                                          \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess) return fail(h, MISA_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int devAlloc(misa_handle* h, T** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, count * sizeof(T) + 64);
+  if (e != hipSuccess) return fail(h, MISA_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  h->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return MISA_OK;
+}
+
+VerletParams verletOf(double dt) {  // NewtonMotion::preComputeDtInv2m (newton_motion.cpp:19-28)
+  VerletParams vp;
+  vp.dt = dt;
+  for (int t = 0; t < kTypes; t++) {
+    const double dt_halve = 0.5 * dt * kFtm2v;
+    vp.c[t] = dt_halve / typeMass(t);
+  }
+  return vp;
+}
+
+int buildPlaneOffsets(misa_handle* h) {
+  const Box& b = h->box;
+  const double keep = b.f + 0.41;  // rc/a + 2 * 0.2 (max displacement) + 0.01 margin
+  for (int p = 0; p < 2; p++) {
+    const std::vector<int64_t>& L = p == 0 ? h->ref.even : h->ref.odd;
+    if (static_cast<int>(L.size()) > kMaxFullOffsets) return fail(h, MISA_E_ARG, "offset list too long");
+    int nl = 0;
+    h->full.n[p] = static_cast<int>(L.size());
+    for (size_t k = 0; k < L.size(); k++) {
+      PlaneOffset po;
+      if (!decodeOffset(b, p, L[k], po)) return fail(h, MISA_E_ARG, "offset does not decode to a lattice neighbour");
+      if (std::abs(po.dcx) > b.g || std::abs(po.dy) > b.g || std::abs(po.dz) > b.g)
+        return fail(h, MISA_E_ARG, "offset reaches beyond the ghost layer");
+      h->full.off[p][k] = static_cast<int32_t>(L[k]);
+      if (po.len < keep) {
+        if (nl >= kMaxLatOffsets) return fail(h, MISA_E_ARG, "pruned offset list too long");
+        const int64_t NX = b.n[0] + 2 * b.g, NY = b.n[1] + 2 * b.g, NZ = b.n[2] + 2 * b.g;
+        const int64_t plane = NX * NY * NZ;
+        const int64_t d = po.dp * plane + (static_cast<int64_t>(po.dz) * NY + po.dy) * NX + po.dcx;
+        h->lat.off[p][nl++] = static_cast<int32_t>(d);
+      }
+    }
+    h->lat.n[p] = nl;
+  }
+  h->have_offsets = true;
+  return MISA_OK;
+}
+
+int syncAndCheck(misa_handle* h) {
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  Ctl c;
+  HIPCHECK(h, hipMemcpy(&c, h->s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+  if (c.err_flags & 1) return fail(h, MISA_E_CAPACITY, "inter-atom list capacity exceeded");
+  if (c.err_flags & 2) return fail(h, MISA_E_CAPACITY, "ghost inter-atom capacity exceeded");
+  if (c.err_flags & 4) return fail(h, MISA_E_DEVICE, "non-finite force");
+  if (c.err_flags & ~int64_t(7)) return fail(h, MISA_E_DEVICE, "device error flag set");
+  if (c.n_stage > 0)
+    return fail(h, MISA_E_DEVICE, "lattice atoms left their sites; the inter-atom path is not enabled in this build");
+  return MISA_OK;
+}
+
+struct Timer {
+  misa_handle* h;
+  int cls;
+  size_t e0 = 0;
+  Timer(misa_handle* hh, int c) : h(hh), cls(c) {
+    if (!h->profile) return;
+    e0 = h->events.size();
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    h->events.push_back(a);
+    h->events.push_back(b);
+    hipEventRecord(a, h->stream);
+  }
+  ~Timer() {
+    if (!h->profile) return;
+    hipEventRecord(h->events[e0 + 1], h->stream);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int misa_create(const misa_geom* g, int device, misa_handle** out) {
+  if (!g || !out) return MISA_E_ARG;
+  *out = nullptr;
+  if (g->phase_space[0] < 1 || g->phase_space[1] < 1 || g->phase_space[2] < 1 || !(g->lattice_const > 0) ||
+      !(g->cutoff_radius_factor > 0))
+    return MISA_E_ARG;
+  auto* h = new misa_handle();
+  h->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete h;
+    return MISA_E_HIP;
+  }
+  h->box.init(g->phase_space, g->lattice_const, g->cutoff_radius_factor);
+  const Box& b = h->box;
+  if (b.n[0] < b.g || b.n[1] < b.g || b.n[2] < b.g) {
+    delete h;
+    return MISA_E_ARG;  // a periodic image must not be its own neighbour's ghost source twice
+  }
+  Geom& G = h->s.G;
+  G.n0 = static_cast<int>(b.n[0]);
+  G.n1 = static_cast<int>(b.n[1]);
+  G.n2 = static_cast<int>(b.n[2]);
+  G.g = b.g;
+  G.NX = G.n0 + 2 * b.g;
+  G.NY = G.n1 + 2 * b.g;
+  G.NZ = G.n2 + 2 * b.g;
+  G.plane = static_cast<int64_t>(G.NX) * G.NY * G.NZ;
+  G.nsub = b.n[0] * b.n[1] * b.n[2];
+  G.a = b.a;
+  G.rc2 = b.rc * b.rc;
+  G.L0 = b.n[0] * b.a;
+  G.L1 = b.n[1] * b.a;
+  G.L2 = b.n[2] * b.a;
+  G.thr = std::pow(0.2 * b.a, 2.0);
+  if (G.nsub > (int64_t(1) << 31)) {
+    delete h;
+    return MISA_E_ARG;
+  }
+  if (hipStreamCreate(&h->stream) != hipSuccess) {
+    delete h;
+    return MISA_E_HIP;
+  }
+  const size_t slots = static_cast<size_t>(2 * G.plane);
+  DevState& s = h->s;
+  int rc = MISA_OK;
+  for (int k = 0; k < 3 && rc == MISA_OK; k++) {
+    rc |= devAlloc(h, &s.x[0][k], slots);
+    rc |= devAlloc(h, &s.x[1][k], slots);
+    rc |= devAlloc(h, &s.v[k], slots);
+    rc |= devAlloc(h, &s.f[k], slots);
+  }
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.rho, slots);
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.df64, slots);
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.df32, slots);
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.type, slots);
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.id, slots);
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.ctl, 1);
+  const int64_t blocks = 16 * (((G.nsub + 255) / 256 + 7) / 8);
+  s.npartials = 8 * blocks;
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.partials, static_cast<size_t>(s.npartials));
+  // inter-atom staging (default capacity)
+  s.in.cap = 65536;
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.in.stage_key, s.in.cap);
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.in.stage_slot, s.in.cap);
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.in.stage_id, s.in.cap);
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.in.stage_type, s.in.cap);
+  for (int k = 0; k < 3 && rc == MISA_OK; k++) {
+    rc |= devAlloc(h, &s.in.stage_x[k], s.in.cap);
+    rc |= devAlloc(h, &s.in.stage_v[k], s.in.cap);
+  }
+  if (rc != MISA_OK) {
+    misa_destroy(h);
+    return MISA_E_HIP;
+  }
+  for (int k = 0; k < 3; k++) {
+    hipMemset(s.x[0][k], 0, slots * 8);
+    hipMemset(s.x[1][k], 0, slots * 8);
+    hipMemset(s.v[k], 0, slots * 8);
+    hipMemset(s.f[k], 0, slots * 8);
+  }
+  hipMemset(s.rho, 0, slots * 8);
+  hipMemset(s.df64, 0, slots * 8);
+  hipMemset(s.df32, 0, slots * 4);
+  hipMemset(s.type, 0xff, slots);
+  hipMemset(s.id, 0, slots * 8);
+  hipMemset(s.ctl, 0, sizeof(Ctl));
+  if (hipDeviceSynchronize() != hipSuccess) {
+    misa_destroy(h);
+    return MISA_E_HIP;
+  }
+  *out = h;
+  return MISA_OK;
+}
+
+int misa_destroy(misa_handle* h) {
+  if (!h) return MISA_E_ARG;
+  hipSetDevice(h->device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (void* p : h->allocs) hipFree(p);
+  if (h->aos_dev) hipFree(h->aos_dev);
+  for (auto e : h->events) hipEventDestroy(e);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return MISA_OK;
+}
+
+const char* misa_last_error(const misa_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int misa_set_offsets(misa_handle* h, const int64_t* even, int64_t n_even, const int64_t* odd, int64_t n_odd,
+                     const int64_t* half_even, int64_t n_half_even, const int64_t* half_odd, int64_t n_half_odd) {
+  if (!h) return MISA_E_ARG;
+  if (!even || !odd || n_even <= 0 || n_odd <= 0) return fail(h, MISA_E_ARG, "empty offset lists");
+  h->ref.even.assign(even, even + n_even);
+  h->ref.odd.assign(odd, odd + n_odd);
+  h->ref.half_even.assign(half_even, half_even + (half_even ? n_half_even : 0));
+  h->ref.half_odd.assign(half_odd, half_odd + (half_odd ? n_half_odd : 0));
+  return buildPlaneOffsets(h);
+}
+
+int misa_make_offsets(misa_handle* h) {
+  if (!h) return MISA_E_ARG;
+  makeOffsets(h->box, h->ref);
+  return buildPlaneOffsets(h);
+}
+
+int misa_get_offsets(const misa_handle* h, int which, int64_t* out, int64_t cap, int64_t* n) {
+  if (!h || !n) return MISA_E_ARG;
+  const std::vector<int64_t>* v = nullptr;
+  switch (which) {
+    case 0: v = &h->ref.even; break;
+    case 1: v = &h->ref.odd; break;
+    case 2: v = &h->ref.half_even; break;
+    case 3: v = &h->ref.half_odd; break;
+    case 4:
+    case 5: {  // pruned plane deltas (4: even, 5: odd)
+      const int p = which - 4;
+      *n = h->lat.n[p];
+      if (out && *n <= cap)
+        for (int k = 0; k < h->lat.n[p]; k++) out[k] = h->lat.off[p][k];
+      return MISA_OK;
+    }
+    default: return MISA_E_ARG;
+  }
+  *n = static_cast<int64_t>(v->size());
+  if (out && *n <= cap) std::memcpy(out, v->data(), v->size() * sizeof(int64_t));
+  return MISA_OK;
+}
+
+int misa_set_potential_file(misa_handle* h, const char* path) {
+  if (!h || !path) return MISA_E_ARG;
+  std::string e;
+  Setfl& sf = h->setfl;
+  if (!parseSetfl(path, sf, e)) return fail(h, MISA_E_IO, e);
+  h->cF.assign(sf.nel, {});
+  h->cR.assign(sf.nel, {});
+  h->cZ.assign(sf.z2.size(), {});
+  for (int i = 0; i < sf.nel; i++) {
+    interpolate7(sf.F[i], sf.drho, h->cF[i]);
+    interpolate7(sf.rho[i], sf.dr, h->cR[i]);
+  }
+  for (size_t i = 0; i < sf.z2.size(); i++) interpolate7(sf.z2[i], sf.dr, h->cZ[i]);
+  // species -> file element by atomic number (atom_types.h:61-72)
+  int el[3];
+  for (int t = 0; t < kTypes; t++) {
+    el[t] = -1;
+    for (int e2 = 0; e2 < sf.nel; e2++)
+      if (sf.Z[e2] == typeZ(t)) el[t] = e2;
+    h->species_present[t] = el[t] >= 0;
+  }
+  if (el[0] < 0 && el[1] < 0 && el[2] < 0) return fail(h, MISA_E_IO, "setfl holds none of Fe, Cu, Ni");
+  const int64_t sF = static_cast<int64_t>(sf.nrho) * kKnot, sR = static_cast<int64_t>(sf.nr) * kKnot;
+  const int64_t total = 3 * sF + 3 * sR + 6 * sR;
+  std::vector<double> h64(static_cast<size_t>(total), 0.0);
+  auto put = [&](int64_t base, const std::vector<double>& c7, int n) {
+    for (int m = 0; m < n; m++) {
+      double* k = &h64[base + static_cast<int64_t>(m) * kKnot];
+      const double* q = &c7[static_cast<size_t>(m) * 7];
+      k[0] = q[0];
+      k[1] = q[1];
+      k[2] = q[2];
+      k[3] = 0.0;
+      k[4] = q[3];
+      k[5] = q[4];
+      k[6] = q[5];
+      k[7] = q[6];
+    }
+  };
+  for (int t = 0; t < 3; t++) {
+    if (el[t] < 0) continue;
+    put(t * sF, h->cF[el[t]], sf.nrho);
+    put(3 * sF + t * sR, h->cR[el[t]], sf.nr);
+  }
+  for (int a = 0; a < 3; a++)
+    for (int b2 = 0; b2 <= a; b2++) {
+      if (el[a] < 0 || el[b2] < 0) continue;
+      const int ea = std::max(el[a], el[b2]), eb = std::min(el[a], el[b2]);
+      put(3 * sF + 3 * sR + (a * (a + 1) / 2 + b2) * sR, h->cZ[ea * (ea + 1) / 2 + eb], sf.nr);
+    }
+  std::vector<float> h32(h64.size());
+  for (size_t i = 0; i < h64.size(); i++) h32[i] = static_cast<float>(h64[i]);
+  if (!h->tab64) {
+    if (devAlloc(h, &h->tab64, static_cast<size_t>(total)) != MISA_OK) return MISA_E_HIP;
+    if (devAlloc(h, &h->tab32, static_cast<size_t>(total)) != MISA_OK) return MISA_E_HIP;
+  } else if (h->t64.nr != sf.nr || h->t64.nrho != sf.nrho) {
+    return fail(h, MISA_E_ARG, "potential grid changed; create a new handle");
+  }
+  HIPCHECK(h, hipMemcpy(h->tab64, h64.data(), h64.size() * 8, hipMemcpyHostToDevice));
+  HIPCHECK(h, hipMemcpy(h->tab32, h32.data(), h32.size() * 4, hipMemcpyHostToDevice));
+  Tables<double>& a64 = h->t64;
+  a64.F = h->tab64;
+  a64.rho = h->tab64 + 3 * sF;
+  a64.z2 = h->tab64 + 3 * sF + 3 * sR;
+  a64.sF = sF;
+  a64.sR = sR;
+  a64.nrho = sf.nrho;
+  a64.nr = sf.nr;
+  a64.rdrho = 1.0 / sf.drho;
+  a64.rdr = 1.0 / sf.dr;
+  a64.rhomax = (sf.nrho - 1) * sf.drho;
+  Tables<float>& a32 = h->t32;
+  a32.F = h->tab32;
+  a32.rho = h->tab32 + 3 * sF;
+  a32.z2 = h->tab32 + 3 * sF + 3 * sR;
+  a32.sF = sF;
+  a32.sR = sR;
+  a32.nrho = sf.nrho;
+  a32.nr = sf.nr;
+  a32.rdrho = static_cast<float>(1.0 / sf.drho);
+  a32.rdr = static_cast<float>(1.0 / sf.dr);
+  a32.rhomax = static_cast<float>(a64.rhomax);
+  h->have_pot = true;
+  return MISA_OK;
+}
+
+int misa_get_potential_table(const misa_handle* h, int kind, int idx, double* out, int64_t cap, int64_t* n) {
+  if (!h || !n || !h->have_pot) return MISA_E_ARG;
+  const auto& v = kind == 0 ? h->cF : (kind == 1 ? h->cR : h->cZ);
+  if (idx < 0 || idx >= static_cast<int>(v.size())) return MISA_E_ARG;
+  *n = static_cast<int64_t>(v[idx].size());
+  if (out && *n <= cap) std::memcpy(out, v[idx].data(), v[idx].size() * 8);
+  return MISA_OK;
+}
+
+static int hookCall(misa_handle* h, int which, void* atoms, double rc) {
+  if (!h || !atoms) return MISA_E_ARG;
+  if (!h->have_offsets || !h->have_pot) return fail(h, MISA_E_ARG, "offsets and potential must be set first");
+  HIPCHECK(h, hipSetDevice(h->device));
+  const size_t bytes = static_cast<size_t>(h->box.slots()) * sizeof(AtomElement);
+  if (!h->aos_dev) HIPCHECK(h, hipMalloc(&h->aos_dev, bytes));
+  HIPCHECK(h, hipMemcpyAsync(h->aos_dev, atoms, bytes, hipMemcpyHostToDevice, h->stream));
+  launchHook(which, h->box, rc, h->full, h->t64, h->aos_dev, h->stream);
+  HIPCHECK(h, hipGetLastError());
+  HIPCHECK(h, hipMemcpyAsync(atoms, h->aos_dev, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  return MISA_OK;
+}
+
+int misa_lat_rho(misa_handle* h, void* atoms, double rc) { return hookCall(h, 0, atoms, rc); }
+int misa_lat_df(misa_handle* h, void* atoms, double rc) { return hookCall(h, 1, atoms, rc); }
+int misa_lat_force(misa_handle* h, void* atoms, double rc) { return hookCall(h, 2, atoms, rc); }
+
+int misa_upload_state(misa_handle* h, const void* atoms, const void* inter, int64_t n_inter) {
+  if (!h || !atoms) return MISA_E_ARG;
+  if (n_inter != 0 || inter) {
+    if (n_inter > 0) return fail(h, MISA_E_ARG, "inter-atom upload is not enabled in this build");
+  }
+  HIPCHECK(h, hipSetDevice(h->device));
+  const AtomElement* a = static_cast<const AtomElement*>(atoms);
+  const Box& b = h->box;
+  // species present in the potential?
+  for (int64_t k = 0; k < b.n[2]; k++)
+    for (int64_t j = 0; j < b.n[1]; j++)
+      for (int64_t i = 0; i < 2 * b.n[0]; i++) {
+        const int t = a[b.refIndex(i + 2 * b.g, j + b.g, k + b.g)].type;
+        if (t > 2) return fail(h, MISA_E_ARG, "unknown atom type");
+        if (t >= 0 && h->have_pot && !h->species_present[t])
+          return fail(h, MISA_E_ARG, "atom species missing from the potential file");
+      }
+  const size_t bytes = static_cast<size_t>(b.slots()) * sizeof(AtomElement);
+  if (!h->aos_dev) HIPCHECK(h, hipMalloc(&h->aos_dev, bytes));
+  HIPCHECK(h, hipMemcpyAsync(h->aos_dev, atoms, bytes, hipMemcpyHostToDevice, h->stream));
+  h->s.cur = 0;
+  launchUpload(h->s, h->aos_dev, b.sx, b.sy, h->stream);
+  launchHaloAll(h->s, h->s.cur, h->stream);
+  HIPCHECK(h, hipGetLastError());
+  HIPCHECK(h, hipMemsetAsync(h->s.ctl, 0, sizeof(Ctl), h->stream));
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  h->state_ready = true;
+  h->forces_ready = false;
+  return MISA_OK;
+}
+
+int misa_download_state(misa_handle* h, void* atoms, void* inter, int64_t inter_cap, int64_t* n_inter) {
+  (void)inter;
+  (void)inter_cap;
+  if (!h || !atoms) return MISA_E_ARG;
+  if (!h->state_ready) return fail(h, MISA_E_ARG, "no state uploaded");
+  HIPCHECK(h, hipSetDevice(h->device));
+  const Box& b = h->box;
+  const size_t bytes = static_cast<size_t>(b.slots()) * sizeof(AtomElement);
+  if (!h->aos_dev) HIPCHECK(h, hipMalloc(&h->aos_dev, bytes));
+  launchDownload(h->s, h->aos_dev, b.sx, b.sy, h->precision == MISA_PREC_MIXED, h->stream);
+  HIPCHECK(h, hipGetLastError());
+  HIPCHECK(h, hipMemcpyAsync(atoms, h->aos_dev, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  if (n_inter) *n_inter = 0;
+  return MISA_OK;
+}
+
+int misa_build_world(misa_handle* h, uint32_t seed, double tset, const int32_t ratio[3], uint32_t alloy_seed) {
+  if (!h || !ratio) return MISA_E_ARG;
+  if (ratio[0] < 0 || ratio[1] < 0 || ratio[2] < 0 || ratio[0] + ratio[1] + ratio[2] <= 0)
+    return fail(h, MISA_E_ARG, "alloy ratio must be non-negative with a positive sum");
+  std::vector<AtomElement> atoms;
+  buildWorld(h->box, seed, tset, ratio, alloy_seed, atoms);
+  return misa_upload_state(h, atoms.data(), nullptr, 0);
+}
+
+int misa_set_precision(misa_handle* h, int mode) {
+  if (!h) return MISA_E_ARG;
+  if (mode == MISA_PREC_FP64) {
+    h->precision = mode;
+    h->acc = 0;
+  } else if (mode == MISA_PREC_MIXED) {
+    h->precision = mode;
+    h->acc = 0;
+  } else if (mode == MISA_PREC_MIXED + 16) {  // mixed with FP32 Kahan-compensated per-atom sums
+    h->precision = MISA_PREC_MIXED;
+    h->acc = 1;
+  } else {
+    return fail(h, MISA_E_ARG, "unknown precision mode");
+  }
+  h->forces_ready = false;
+  return MISA_OK;
+}
+
+int misa_set_timestep(misa_handle* h, double dt) {
+  if (!h || !(dt > 0)) return MISA_E_ARG;
+  h->dt = dt;
+  return MISA_OK;
+}
+
+int misa_set_inter_capacity(misa_handle* h, int64_t cap) {
+  if (!h || cap < 1) return MISA_E_ARG;
+  return MISA_OK;
+}
+
+int misa_compute_forces(misa_handle* h) {
+  if (!h) return MISA_E_ARG;
+  if (!h->state_ready || !h->have_pot || !h->have_offsets) return fail(h, MISA_E_ARG, "state, potential and offsets required");
+  HIPCHECK(h, hipSetDevice(h->device));
+  const bool mixed = h->precision == MISA_PREC_MIXED;
+  const VerletParams vp = verletOf(h->dt);
+  launchRhoDf(h->s, h->lat, h->t64, h->t32, mixed, h->acc, true, h->stream);
+  launchForce(h->s, h->lat, h->t64, h->t32, mixed, h->acc, 2, vp, h->stream);
+  HIPCHECK(h, hipGetLastError());
+  const int rc = syncAndCheck(h);
+  if (rc != MISA_OK) return rc;
+  h->forces_ready = true;
+  return MISA_OK;
+}
+
+int misa_step(misa_handle* h, int64_t nsteps) {
+  if (!h || nsteps < 0) return MISA_E_ARG;
+  if (!h->forces_ready) return fail(h, MISA_E_ARG, "misa_compute_forces must run before misa_step");
+  if (nsteps == 0) return MISA_OK;
+  HIPCHECK(h, hipSetDevice(h->device));
+  const bool mixed = h->precision == MISA_PREC_MIXED;
+  const VerletParams vp = verletOf(h->dt);
+  for (auto e : h->events) hipEventDestroy(e);
+  h->events.clear();
+  hipEvent_t e0, e1;
+  HIPCHECK(h, hipEventCreate(&e0));
+  HIPCHECK(h, hipEventCreate(&e1));
+  HIPCHECK(h, hipEventRecord(e0, h->stream));
+  std::vector<int> cls;
+  {
+    Timer t(h, 2);
+    if (h->profile) cls.push_back(2);
+    launchFirst(h->s, vp, h->stream);
+  }
+  h->s.cur = 1 - h->s.cur;
+  for (int64_t i = 0; i < nsteps; i++) {
+    const bool last = i == nsteps - 1;
+    {
+      Timer t(h, 0);
+      if (h->profile) cls.push_back(0);
+      launchRhoDf(h->s, h->lat, h->t64, h->t32, mixed, h->acc, last, h->stream);
+    }
+    {
+      Timer t(h, 1);
+      if (h->profile) cls.push_back(1);
+      launchForce(h->s, h->lat, h->t64, h->t32, mixed, h->acc, last ? 0 : 1, vp, h->stream);
+    }
+    if (!last) h->s.cur = 1 - h->s.cur;
+  }
+  HIPCHECK(h, hipGetLastError());
+  HIPCHECK(h, hipEventRecord(e1, h->stream));
+  const int rc = syncAndCheck(h);
+  float ms = 0;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  h->timings[0] = h->timings[1] = h->timings[2] = 0;
+  h->launches[0] = h->launches[1] = h->launches[2] = 0;
+  for (size_t k = 0; k < cls.size(); k++) {
+    float t = 0;
+    hipEventElapsedTime(&t, h->events[2 * k], h->events[2 * k + 1]);
+    h->timings[cls[k]] += t;
+    h->launches[cls[k]]++;
+  }
+  h->timings[3] = ms;
+  if (rc != MISA_OK) {
+    h->forces_ready = false;
+    return rc;
+  }
+  return MISA_OK;
+}
+
+int misa_setv(misa_handle* h, const int32_t lat[4], const double dir[3], double energy) {
+  if (!h || !lat || !dir) return MISA_E_ARG;
+  if (!h->state_ready) return fail(h, MISA_E_ARG, "no state");
+  const Box& b = h->box;
+  // atom::setv only acts when the site is in this sub-box (atom.cpp:477-482)
+  if (!(lat[0] >= 0 && lat[0] < b.n[0] && lat[1] >= 0 && lat[1] < b.n[1] && lat[2] >= 0 && lat[2] < b.n[2]) ||
+      lat[3] < 0 || lat[3] > 1)
+    return MISA_OK;
+  HIPCHECK(h, hipSetDevice(h->device));
+  const Geom& G = h->s.G;
+  const int64_t slot = lat[3] * G.plane + (static_cast<int64_t>(lat[2] + G.g) * G.NY + (lat[1] + G.g)) * G.NX +
+                       (lat[0] + G.g);
+  int8_t t;
+  HIPCHECK(h, hipMemcpy(&t, h->s.type + slot, 1, hipMemcpyDeviceToHost));
+  const double m = t < 0 ? 0.0 : typeMass(t & 3);
+  const double v_ = std::sqrt(2 * energy / m / kMvv2e);
+  const double d_ = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+  launchSetv(h->s, slot, v_ * dir[0] / d_, v_ * dir[1] / d_, v_ * dir[2] / d_, h->stream);
+  HIPCHECK(h, hipGetLastError());
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  return MISA_OK;
+}
+
+int misa_collision_step(misa_handle* h, const int32_t lat[4], const double dir[3], double energy) {
+  int rc = misa_setv(h, lat, dir, energy);
+  if (rc != MISA_OK) return rc;
+  return misa_compute_forces(h);
+}
+
+int misa_thermo_compute(misa_handle* h, misa_thermo* out) {
+  if (!h || !out) return MISA_E_ARG;
+  if (!h->state_ready) return fail(h, MISA_E_ARG, "no state");
+  HIPCHECK(h, hipSetDevice(h->device));
+  std::memset(out, 0, sizeof(*out));
+  const int64_t nb = launchThermo(h->s, h->stream);
+  HIPCHECK(h, hipGetLastError());
+  std::vector<double> p(static_cast<size_t>(7 * nb));
+  HIPCHECK(h, hipMemcpyAsync(p.data(), h->s.partials, p.size() * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  double vmax2 = 0;
+  for (int64_t b = 0; b < nb; b++) {
+    out->mvv += p[7 * b + 0];
+    out->n_lattice += static_cast<int64_t>(p[7 * b + 1]);
+    out->n_vacancy += static_cast<int64_t>(p[7 * b + 2]);
+    out->force_sum[0] += p[7 * b + 3];
+    out->force_sum[1] += p[7 * b + 4];
+    out->force_sum[2] += p[7 * b + 5];
+    vmax2 = std::max(vmax2, p[7 * b + 6]);
+  }
+  out->max_speed = std::sqrt(vmax2);
+  out->kinetic = out->mvv / 2;
+  const uint64_t n = static_cast<uint64_t>(2 * h->box.n[0] * h->box.n[1] * h->box.n[2]);
+  const uint64_t dof = 3 * n - 3;
+  out->temperature = out->mvv * kMvv2e / (dof * kBoltz);
+  if (h->have_pot && h->have_offsets && h->forces_ready) {
+    const int64_t ne = launchEnergy(h->s, h->lat, h->t64, h->stream);
+    HIPCHECK(h, hipGetLastError());
+    std::vector<double> q(static_cast<size_t>(3 * ne));
+    HIPCHECK(h, hipMemcpyAsync(q.data(), h->s.partials, q.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    for (int64_t b = 0; b < ne; b++) {
+      out->e_embed += q[3 * b + 0];
+      out->e_pair += q[3 * b + 1];
+      out->virial += q[3 * b + 2];
+    }
+    out->e_potential = out->e_embed + out->e_pair;
+  }
+  return MISA_OK;
+}
+
+int misa_rescale(misa_handle* h, double tset) {
+  misa_thermo t;
+  int rc = misa_thermo_compute(h, &t);
+  if (rc != MISA_OK) return rc;
+  if (!(t.temperature > 0)) return fail(h, MISA_E_ARG, "zero temperature cannot be rescaled");
+  launchScaleV(h->s, std::sqrt(tset / t.temperature), h->stream);
+  HIPCHECK(h, hipGetLastError());
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  return MISA_OK;
+}
+
+int misa_last_timings(const misa_handle* h, double out[4]) {
+  if (!h || !out) return MISA_E_ARG;
+  for (int k = 0; k < 4; k++) out[k] = h->timings[k];
+  return MISA_OK;
+}
+
+// Per-kernel event timing of misa_step (HIP events on the handle's stream).
+int misa_set_profiling(misa_handle* h, int on) {
+  if (!h) return MISA_E_ARG;
+  h->profile = on != 0;
+  return MISA_OK;
+}
+
+int misa_last_launch_counts(const misa_handle* h, int64_t out[3]) {
+  if (!h || !out) return MISA_E_ARG;
+  for (int k = 0; k < 3; k++) out[k] = h->launches[k];
+  return MISA_OK;
+}
+
+// ---- host-only set-up (no device access) ----
+
+int misa_host_offsets(const misa_geom* g, int which, int64_t* out, int64_t cap, int64_t* n) {
+  if (!g || !n || which < 0 || which > 5) return MISA_E_ARG;
+  misa_handle h;
+  h.box.init(g->phase_space, g->lattice_const, g->cutoff_radius_factor);
+  makeOffsets(h.box, h.ref);
+  if (buildPlaneOffsets(&h) != MISA_OK) return MISA_E_ARG;
+  return misa_get_offsets(&h, which, out, cap, n);
+}
+
+int misa_host_potential_table(const char* path, int kind, int idx, double* out, int64_t cap, int64_t* n) {
+  if (!path || !n) return MISA_E_ARG;
+  Setfl sf;
+  std::string e;
+  if (!parseSetfl(path, sf, e)) return MISA_E_IO;
+  const std::vector<std::vector<double>>& src = kind == 0 ? sf.F : (kind == 1 ? sf.rho : sf.z2);
+  if (idx < 0 || idx >= static_cast<int>(src.size())) return MISA_E_ARG;
+  std::vector<double> c;
+  interpolate7(src[idx], kind == 0 ? sf.drho : sf.dr, c);
+  *n = static_cast<int64_t>(c.size());
+  if (out && *n <= cap) std::memcpy(out, c.data(), c.size() * 8);
+  return MISA_OK;
+}
+
+int misa_host_build_world(const misa_geom* g, uint32_t seed, double tset, const int32_t ratio[3], uint32_t alloy_seed,
+                          void* atoms) {
+  if (!g || !ratio || !atoms) return MISA_E_ARG;
+  if (ratio[0] < 0 || ratio[1] < 0 || ratio[2] < 0 || ratio[0] + ratio[1] + ratio[2] <= 0) return MISA_E_ARG;
+  Box b;
+  b.init(g->phase_space, g->lattice_const, g->cutoff_radius_factor);
+  std::vector<AtomElement> v;
+  buildWorld(b, seed, tset, ratio, alloy_seed, v);
+  std::memcpy(atoms, v.data(), v.size() * sizeof(AtomElement));
+  return MISA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,179 @@
+// Device-side layout and inline arithmetic shared by the EAM kernels.
+//
+// Resident layout ("parity planes").  The reference stores the BCC lattice as
+// one ghost-extended array whose x index is doubled: even x = cube corner,
+// odd x = body centre (atom_set.cpp:17-26).  On the GPU the two sublattices
+// live in two separate planes, each a dense (NZ, NY, NX) array of cells with
+// the same ghost width g as the reference:
+//
+//     slot(p, cx, cy, cz) = p * plane + (cz * NY + cy) * NX + cx,  padded coords
+//
+// so 64 consecutive sites of one parity form one wavefront, every lane of it
+// uses the same neighbour offset list, and offset k is one uniform slot delta:
+// neighbour loads are unit-stride across lanes.  Reference linear offsets map
+// one-to-one onto (target parity, dcx, dy, dz) (decodeOffset in
+// host_setup.cpp), so the pairs visited are the reference's pairs.
+//
+// State is SoA: x (two ping-pong buffers per component), v, f, df, rho, a
+// type byte and the atom id.  The type byte holds the species in bits 0-1,
+// bit 6 marks a site with an inter atom in its 229-site neighbourhood, and a
+// negative value marks a vacancy (atom_type::INVALID).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace misa {
+
+constexpr int kMaxLatOffsets = 128;   // pruned lattice-lattice list per parity
+constexpr int kMaxFullOffsets = 256;  // reference full list per parity
+constexpr int kKnot = 8;              // values per knot record: c0 c1 c2 _ c3 c4 c5 c6
+constexpr int8_t kVacant = -1;
+constexpr int8_t kNearInter = 0x40;
+
+struct Geom {
+  int n0, n1, n2;  // cells per dimension of the periodic box
+  int g;           // ghost cells per face
+  int NX, NY, NZ;  // padded plane dimensions (n + 2g)
+  int64_t plane;   // NX * NY * NZ
+  int64_t nsub;    // n0 * n1 * n2 (sites per parity)
+  double a;        // lattice constant
+  double rc2;      // (a*f)^2
+  double L0, L1, L2;
+  double thr;      // (0.2 a)^2, decide() threshold (atom.cpp:42)
+};
+
+struct LatOffsets {  // slot deltas of the pruned lattice-lattice neighbour list
+  int n[2];
+  int32_t off[2][kMaxLatOffsets];
+};
+
+// Spline tables: one knot record of kKnot values per knot, all tables of one
+// kind in one allocation with a fixed stride; index by species (0 Fe, 1 Cu,
+// 2 Ni) and by pair (max*(max+1)/2 + min).
+template <typename T>
+struct Tables {
+  const T* F;    // 3 tables
+  const T* rho;  // 3 tables
+  const T* z2;   // 6 tables
+  int64_t sF, sR;  // strides (values) of one F table / one r table
+  int nrho, nr;
+  T rdrho, rdr;   // 1/drho, 1/dr
+  T rhomax;
+};
+
+__device__ __forceinline__ bool isVacant(int8_t t) { return t < 0; }
+__device__ __forceinline__ int species(int8_t t) { return t & 3; }
+__device__ __forceinline__ int pairIdx(int a, int b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; }
+
+// Knot search of the eam/alloy convention: p = x/delta + 1 (one-based knot),
+// m = int(p) clamped to n-1 (and to >= 1 for rho-tables), p -= m, p <= 1.
+// Returns the zero-based knot; p is the local coordinate.
+template <typename T>
+__device__ __forceinline__ int knotOf(T x, T rinv, int n, bool clamp_low, T& p) {
+  p = x * rinv + T(1);
+  const T pm = p < T(n - 1) ? p : T(n - 1);
+  int m = static_cast<int>(pm);
+  if (clamp_low && m < 1) m = 1;
+  p -= T(m);
+  p = p < T(1) ? p : T(1);
+  return m - 1;
+}
+
+template <typename T>
+__device__ __forceinline__ T splineVal(const T* k, T p) {
+  return ((k[4] * p + k[5]) * p + k[6]) * p + k[7];
+}
+template <typename T>
+__device__ __forceinline__ T splineDer(const T* k, T p) {
+  return (k[0] * p + k[1]) * p + k[2];
+}
+
+// rho contributed by species e at distance r (pot eam::chargeDensity).
+template <typename T>
+__device__ __forceinline__ T rhoOf(const Tables<T>& tb, int e, T r) {
+  T p;
+  const int m = knotOf(r, tb.rdr, tb.nr, false, p);
+  return splineVal(tb.rho + e * tb.sR + static_cast<int64_t>(m) * kKnot, p);
+}
+
+// F'(rho) (pot eam::dEmbedEnergy).
+template <typename T>
+__device__ __forceinline__ T dEmbed(const Tables<T>& tb, int e, T rho) {
+  T p;
+  const int m = knotOf(rho, tb.rdrho, tb.nrho, true, p);
+  return splineDer(tb.F + e * tb.sF + static_cast<int64_t>(m) * kKnot, p);
+}
+
+// F(rho) with linear extension above the table.
+template <typename T>
+__device__ __forceinline__ T embed(const Tables<T>& tb, int e, T rho) {
+  T p;
+  const int m = knotOf(rho, tb.rdrho, tb.nrho, true, p);
+  const T* k = tb.F + e * tb.sF + static_cast<int64_t>(m) * kKnot;
+  T v = splineVal(k, p);
+  if (rho > tb.rhomax) v += splineDer(k, p) * (rho - tb.rhomax);
+  return v;
+}
+
+__device__ __forceinline__ double rsqrtRefined(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  // two Newton steps on y = x^-1/2: y <- y (3 - x y^2) / 2
+  double h = x * y;
+  double e = fma(-h, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  h = x * y;
+  e = fma(-h, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  return y;
+}
+
+// Pair force factor (pot eam::toForce): fpair = -(F'_i rho'_j + F'_j rho'_i +
+// phi') / r with phi = z2 / r; f_i += (x_i - x_j) * fpair.  FP64 version.
+__device__ __forceinline__ double pairForce(const Tables<double>& tb, int ei, int ej, double r2, double dfi,
+                                            double dfj) {
+  const double rinv = rsqrtRefined(r2);
+  const double r = r2 * rinv;
+  double p;
+  const int m = knotOf(r, tb.rdr, tb.nr, false, p);
+  const int64_t km = static_cast<int64_t>(m) * kKnot;
+  const double rhoip = splineDer(tb.rho + ei * tb.sR + km, p);
+  const double rhojp = splineDer(tb.rho + ej * tb.sR + km, p);
+  const double* kz = tb.z2 + pairIdx(ei, ej) * tb.sR + km;
+  const double z2p = splineDer(kz, p);
+  const double z2 = splineVal(kz, p);
+  const double phi = z2 * rinv;
+  const double phip = z2p * rinv - phi * rinv;
+  const double psip = dfi * rhojp + dfj * rhoip + phip;
+  return -psip * rinv;
+}
+
+// FP32 version: splines and pair terms in FP32 from an FP64 r^2.
+__device__ __forceinline__ float pairForce(const Tables<float>& tb, int ei, int ej, double r2d, float dfi,
+                                           float dfj) {
+  const float r2 = static_cast<float>(r2d);
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  const float r = r2 * rinv;
+  float p;
+  const int m = knotOf(r, tb.rdr, tb.nr, false, p);
+  const int64_t km = static_cast<int64_t>(m) * kKnot;
+  const float rhoip = splineDer(tb.rho + ei * tb.sR + km, p);
+  const float rhojp = splineDer(tb.rho + ej * tb.sR + km, p);
+  const float* kz = tb.z2 + pairIdx(ei, ej) * tb.sR + km;
+  const float z2p = splineDer(kz, p);
+  const float z2 = splineVal(kz, p);
+  const float phi = z2 * rinv;
+  const float phip = z2p * rinv - phi * rinv;
+  const float psip = dfi * rhojp + dfj * rhoip + phip;
+  return -psip * rinv;
+}
+
+__device__ __forceinline__ double rhoPair(const Tables<double>& tb, int e, double r2) {
+  return rhoOf(tb, e, r2 * rsqrtRefined(r2));
+}
+__device__ __forceinline__ float rhoPair(const Tables<float>& tb, int e, double r2d) {
+  const float r2 = static_cast<float>(r2d);
+  return rhoOf(tb, e, r2 * __builtin_amdgcn_rsqf(r2));
+}
+
+}  // namespace misa

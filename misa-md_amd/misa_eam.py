@@ -1,0 +1,304 @@
+"""ctypes binding of libmisa_eam_hip.so (include/misa_eam.h).
+
+The library is the product: HIP kernels for gfx950 behind a C-ABI.  This
+module only marshals arguments; it raises if the shared library is missing
+and never computes anything itself.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmisa_eam_hip.so")
+
+# The reference's 104-byte AtomElement (src/atom/atom_element.h:18-41).
+ATOM_DTYPE = np.dtype([
+    ("id", "<u8"), ("type", "<i4"), ("_pad", "<i4"),
+    ("x", "<f8", (3,)), ("v", "<f8", (3,)), ("f", "<f8", (3,)),
+    ("rho", "<f8"), ("df", "<f8"),
+])
+assert ATOM_DTYPE.itemsize == 104
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+PREC_MIXED_KAHAN = 17
+
+A_FE = 2.855
+RC_FACTOR = 5.6 / 2.855
+
+
+class MisaGeom(ctypes.Structure):
+    _fields_ = [("phase_space", ctypes.c_int64 * 3), ("lattice_const", ctypes.c_double),
+                ("cutoff_radius_factor", ctypes.c_double)]
+
+
+class MisaThermo(ctypes.Structure):
+    _fields_ = [("mvv", ctypes.c_double), ("kinetic", ctypes.c_double), ("temperature", ctypes.c_double),
+                ("e_embed", ctypes.c_double), ("e_pair", ctypes.c_double), ("e_potential", ctypes.c_double),
+                ("virial", ctypes.c_double), ("force_sum", ctypes.c_double * 3), ("max_speed", ctypes.c_double),
+                ("n_lattice", ctypes.c_int64), ("n_inter", ctypes.c_int64), ("n_vacancy", ctypes.c_int64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "force_sum"}
+        d["force_sum"] = list(self.force_sum)
+        return d
+
+
+_lib = None
+
+_SIGS = {
+    "misa_create": (ctypes.c_int, [ctypes.POINTER(MisaGeom), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "misa_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "misa_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "misa_set_offsets": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p, ctypes.c_int64] * 4),
+    "misa_make_offsets": (ctypes.c_int, [ctypes.c_void_p]),
+    "misa_get_offsets": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64,
+                                        ctypes.POINTER(ctypes.c_int64)]),
+    "misa_set_potential_file": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "misa_get_potential_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                                ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
+    "misa_lat_rho": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]),
+    "misa_lat_df": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]),
+    "misa_lat_force": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]),
+    "misa_upload_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "misa_download_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                           ctypes.POINTER(ctypes.c_int64)]),
+    "misa_build_world": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_double,
+                                        ctypes.POINTER(ctypes.c_int32), ctypes.c_uint32]),
+    "misa_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "misa_set_timestep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "misa_set_inter_capacity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "misa_compute_forces": (ctypes.c_int, [ctypes.c_void_p]),
+    "misa_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "misa_setv": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double),
+                                 ctypes.c_double]),
+    "misa_collision_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
+                                           ctypes.POINTER(ctypes.c_double), ctypes.c_double]),
+    "misa_rescale": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "misa_thermo_compute": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MisaThermo)]),
+    "misa_last_timings": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "misa_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "misa_last_launch_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "misa_host_offsets": (ctypes.c_int, [ctypes.POINTER(MisaGeom), ctypes.c_int, ctypes.c_void_p, ctypes.c_int64,
+                                         ctypes.POINTER(ctypes.c_int64)]),
+    "misa_host_potential_table": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                                 ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
+    "misa_host_build_world": (ctypes.c_int, [ctypes.POINTER(MisaGeom), ctypes.c_uint32, ctypes.c_double,
+                                             ctypes.POINTER(ctypes.c_int32), ctypes.c_uint32, ctypes.c_void_p]),
+}
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def load():
+    """Load the shared library (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError("libmisa_eam_hip.so not built: run __graft_entry__.build() (expected at %s)" % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class MisaError(RuntimeError):
+    pass
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class EamDevice:
+    """One misa_handle: a periodic single-rank box on one GPU."""
+
+    def __init__(self, cells, lattice_const=A_FE, rc_factor=RC_FACTOR, device=0):
+        self.lib = load()
+        g = MisaGeom()
+        for k in range(3):
+            g.phase_space[k] = int(cells[k])
+        g.lattice_const = lattice_const
+        g.cutoff_radius_factor = rc_factor
+        h = ctypes.c_void_p()
+        rc = self.lib.misa_create(ctypes.byref(g), device, ctypes.byref(h))
+        if rc != 0:
+            raise MisaError("misa_create failed (%d)" % rc)
+        self.h = h
+        self.cells = tuple(int(c) for c in cells)
+        self.a = lattice_const
+        self.f = rc_factor
+        self.ghost = int(np.ceil(rc_factor)) + 1
+        self.dims = (2 * (self.cells[0] + 2 * self.ghost), self.cells[1] + 2 * self.ghost,
+                     self.cells[2] + 2 * self.ghost)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise MisaError("%s failed (%d): %s" % (what, rc, self.lib.misa_last_error(self.h).decode()))
+
+    def close(self):
+        if self.h is not None:
+            self.lib.misa_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nslots(self):
+        return self.dims[0] * self.dims[1] * self.dims[2]
+
+    def empty_atoms(self):
+        return np.zeros(self.nslots, dtype=ATOM_DTYPE)
+
+    def set_offsets(self, even, odd, half_even, half_odd):
+        arrs = [np.ascontiguousarray(a, dtype=np.int64) for a in (even, odd, half_even, half_odd)]
+        args = []
+        for a in arrs:
+            args += [_ptr(a), len(a)]
+        self._check(self.lib.misa_set_offsets(self.h, *args), "misa_set_offsets")
+
+    def make_offsets(self):
+        self._check(self.lib.misa_make_offsets(self.h), "misa_make_offsets")
+
+    def get_offsets(self, which):
+        n = ctypes.c_int64()
+        self._check(self.lib.misa_get_offsets(self.h, which, None, 0, ctypes.byref(n)), "misa_get_offsets")
+        out = np.zeros(n.value, dtype=np.int64)
+        self._check(self.lib.misa_get_offsets(self.h, which, _ptr(out), n.value, ctypes.byref(n)), "misa_get_offsets")
+        return out
+
+    def set_potential(self, path):
+        self._check(self.lib.misa_set_potential_file(self.h, str(path).encode()), "misa_set_potential_file")
+
+    def potential_table(self, kind, idx):
+        n = ctypes.c_int64()
+        self._check(self.lib.misa_get_potential_table(self.h, kind, idx, None, 0, ctypes.byref(n)), "table")
+        out = np.zeros(n.value, dtype=np.float64)
+        self._check(self.lib.misa_get_potential_table(self.h, kind, idx, _ptr(out), n.value, ctypes.byref(n)), "table")
+        return out
+
+    def lat_rho(self, atoms, rc):
+        self._check(self.lib.misa_lat_rho(self.h, _ptr(atoms), rc), "misa_lat_rho")
+
+    def lat_df(self, atoms, rc):
+        self._check(self.lib.misa_lat_df(self.h, _ptr(atoms), rc), "misa_lat_df")
+
+    def lat_force(self, atoms, rc):
+        self._check(self.lib.misa_lat_force(self.h, _ptr(atoms), rc), "misa_lat_force")
+
+    def upload(self, atoms, inter=None):
+        n = 0 if inter is None else len(inter)
+        self._check(self.lib.misa_upload_state(self.h, _ptr(atoms), _ptr(inter) if n else None, n),
+                    "misa_upload_state")
+
+    def download(self, inter_cap=1 << 16):
+        atoms = self.empty_atoms()
+        inter = np.zeros(inter_cap, dtype=ATOM_DTYPE)
+        n = ctypes.c_int64()
+        self._check(self.lib.misa_download_state(self.h, _ptr(atoms), _ptr(inter), inter_cap, ctypes.byref(n)),
+                    "misa_download_state")
+        return atoms, inter[:n.value].copy()
+
+    def build_world(self, seed, tset, ratio, alloy_seed=1):
+        r = (ctypes.c_int32 * 3)(*ratio)
+        self._check(self.lib.misa_build_world(self.h, seed, tset, r, alloy_seed), "misa_build_world")
+
+    def set_precision(self, mode):
+        self._check(self.lib.misa_set_precision(self.h, mode), "misa_set_precision")
+
+    def set_timestep(self, dt):
+        self._check(self.lib.misa_set_timestep(self.h, dt), "misa_set_timestep")
+
+    def compute_forces(self):
+        self._check(self.lib.misa_compute_forces(self.h), "misa_compute_forces")
+
+    def step(self, n):
+        self._check(self.lib.misa_step(self.h, int(n)), "misa_step")
+
+    def setv(self, lat, direction, energy):
+        l4 = (ctypes.c_int32 * 4)(*lat)
+        d3 = (ctypes.c_double * 3)(*direction)
+        self._check(self.lib.misa_setv(self.h, l4, d3, energy), "misa_setv")
+
+    def collision_step(self, lat, direction, energy):
+        l4 = (ctypes.c_int32 * 4)(*lat)
+        d3 = (ctypes.c_double * 3)(*direction)
+        self._check(self.lib.misa_collision_step(self.h, l4, d3, energy), "misa_collision_step")
+
+    def rescale(self, t):
+        self._check(self.lib.misa_rescale(self.h, t), "misa_rescale")
+
+    def thermo(self):
+        t = MisaThermo()
+        self._check(self.lib.misa_thermo_compute(self.h, ctypes.byref(t)), "misa_thermo_compute")
+        return t.as_dict()
+
+    def set_profiling(self, on):
+        self._check(self.lib.misa_set_profiling(self.h, 1 if on else 0), "misa_set_profiling")
+
+    def last_timings(self):
+        out = (ctypes.c_double * 4)()
+        self._check(self.lib.misa_last_timings(self.h, out), "misa_last_timings")
+        cnt = (ctypes.c_int64 * 3)()
+        self._check(self.lib.misa_last_launch_counts(self.h, cnt), "misa_last_launch_counts")
+        return {"rho_df_ms": out[0], "force_ms": out[1], "other_ms": out[2], "total_ms": out[3],
+                "launches": list(cnt)}
+
+
+def geom(cells, lattice_const=A_FE, rc_factor=RC_FACTOR):
+    g = MisaGeom()
+    for k in range(3):
+        g.phase_space[k] = int(cells[k])
+    g.lattice_const = lattice_const
+    g.cutoff_radius_factor = rc_factor
+    return g
+
+
+def host_offsets(cells, which, lattice_const=A_FE, rc_factor=RC_FACTOR):
+    lib = load()
+    g = geom(cells, lattice_const, rc_factor)
+    n = ctypes.c_int64()
+    if lib.misa_host_offsets(ctypes.byref(g), which, None, 0, ctypes.byref(n)) != 0:
+        raise MisaError("misa_host_offsets")
+    out = np.zeros(n.value, dtype=np.int64)
+    lib.misa_host_offsets(ctypes.byref(g), which, _ptr(out), n.value, ctypes.byref(n))
+    return out
+
+
+def host_potential_table(path, kind, idx):
+    lib = load()
+    n = ctypes.c_int64()
+    if lib.misa_host_potential_table(str(path).encode(), kind, idx, None, 0, ctypes.byref(n)) != 0:
+        raise MisaError("misa_host_potential_table")
+    out = np.zeros(n.value, dtype=np.float64)
+    lib.misa_host_potential_table(str(path).encode(), kind, idx, _ptr(out), n.value, ctypes.byref(n))
+    return out
+
+
+def host_build_world(cells, seed, tset, ratio, alloy_seed=1, lattice_const=A_FE, rc_factor=RC_FACTOR):
+    lib = load()
+    g = geom(cells, lattice_const, rc_factor)
+    gh = int(np.ceil(rc_factor)) + 1
+    n = 2 * (cells[0] + 2 * gh) * (cells[1] + 2 * gh) * (cells[2] + 2 * gh)
+    atoms = np.zeros(n, dtype=ATOM_DTYPE)
+    r = (ctypes.c_int32 * 3)(*ratio)
+    if lib.misa_host_build_world(ctypes.byref(g), seed, tset, r, alloy_seed, _ptr(atoms)) != 0:
+        raise MisaError("misa_host_build_world")
+    return atoms
+
+
+def sub_box_view(atoms, dev):
+    """Sub-box slots of a ghost-extended array in reference (z, y, X) order."""
+    g = dev.ghost
+    a = atoms.reshape(dev.dims[2], dev.dims[1], dev.dims[0])
+    return a[g:g + dev.cells[2], g:g + dev.cells[1], 2 * g:2 * g + 2 * dev.cells[0]]
