@@ -242,8 +242,15 @@ __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T>
     dfi = dEmbed(tb, species(ti), static_cast<T>(rho));
   }
   df[s] = dfi;
-  if (rho_out) rho_out[s] = rho;
-  forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) { df[q] = dfi; });
+  if (rho_out) {
+    rho_out[s] = rho;
+    forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) {
+      df[q] = dfi;
+      rho_out[q] = rho;
+    });
+  } else {
+    forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) { df[q] = dfi; });
+  }
 }
 
 // ---------------------------------------------------------------------------
