@@ -103,8 +103,33 @@ int buildPlaneOffsets(misa_handle* h) {
     }
     h->lat.n[p] = nl;
   }
+  // non-negative form for the buffer loads: sh = off - lo, lo = most negative delta
+  int32_t lo = 0;
+  for (int p = 0; p < 2; p++)
+    for (int k = 0; k < h->lat.n[p]; k++) lo = std::min(lo, h->lat.off[p][k]);
+  h->lat.lo = lo;
+  for (int p = 0; p < 2; p++)
+    for (int k = 0; k < h->lat.n[p]; k++) h->lat.sh[p][k] = static_cast<uint32_t>(h->lat.off[p][k] - lo);
   h->have_offsets = true;
   return MISA_OK;
+}
+
+// Traversal of the site kernels: z-slab per XCD, y-tiles of up to 16 rows.
+void makeWalk(Geom& G) {
+  Walk& W = G.W;
+  W.ty = 1;
+  for (int t = std::min(16, G.n1); t >= 1; t--)
+    if (G.n1 % t == 0) {
+      W.ty = t;
+      break;
+    }
+  int64_t bmax = 0;
+  for (int c = 0; c <= 8; c++) W.zlo[c] = static_cast<int>((static_cast<int64_t>(c) * G.n2) / 8);
+  for (int c = 0; c < 8; c++) {
+    const int64_t sites = static_cast<int64_t>(W.zlo[c + 1] - W.zlo[c]) * G.n1 * G.n0;
+    bmax = std::max(bmax, (sites + 255) / 256);
+  }
+  W.bpx = bmax;
 }
 
 int syncAndCheck(misa_handle* h) {
@@ -178,7 +203,9 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
   G.L1 = b.n[1] * b.a;
   G.L2 = b.n[2] * b.a;
   G.thr = std::pow(0.2 * b.a, 2.0);
-  if (G.nsub > (int64_t(1) << 31)) {
+  makeWalk(G);
+  // buffer loads address the planes with 32-bit byte offsets (site + shifted delta)
+  if (G.nsub > (int64_t(1) << 31) || 4 * G.plane * 8 >= (int64_t(1) << 31)) {
     delete h;
     return MISA_E_ARG;
   }
@@ -201,7 +228,7 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
   if (rc == MISA_OK) rc |= devAlloc(h, &s.type, slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.id, slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.ctl, 1);
-  const int64_t blocks = 16 * (((G.nsub + 255) / 256 + 7) / 8);
+  const int64_t blocks = 16 * G.W.bpx;
   s.npartials = 8 * blocks;
   if (rc == MISA_OK) rc |= devAlloc(h, &s.partials, static_cast<size_t>(s.npartials));
   // inter-atom staging (default capacity)
@@ -315,7 +342,10 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
   }
   if (el[0] < 0 && el[1] < 0 && el[2] < 0) return fail(h, MISA_E_IO, "setfl holds none of Fe, Cu, Ni");
   const int64_t sF = static_cast<int64_t>(sf.nrho) * kKnot, sR = static_cast<int64_t>(sf.nr) * kKnot;
-  const int64_t total = 3 * sF + 3 * sR + 6 * sR;
+  const int64_t nr = sf.nr;
+  const int64_t oRhoV = 3 * sF + 9 * sR;            // [3][nr][4] density value cubics
+  const int64_t oPair = oRhoV + 3 * nr * kRhoRec;  // [9][nr][16] force records per ordered pair
+  const int64_t total = oPair + 9 * nr * kPairRec;
   std::vector<double> h64(static_cast<size_t>(total), 0.0);
   auto put = [&](int64_t base, const std::vector<double>& c7, int n) {
     for (int m = 0; m < n; m++) {
@@ -342,6 +372,29 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
       const int ea = std::max(el[a], el[b2]), eb = std::min(el[a], el[b2]);
       put(3 * sF + 3 * sR + (a * (a + 1) / 2 + b2) * sR, h->cZ[ea * (ea + 1) / 2 + eb], sf.nr);
     }
+  // packed records: the same coefficients, regrouped for one gather per pair
+  for (int t = 0; t < 3; t++) {
+    if (el[t] < 0) continue;
+    const std::vector<double>& R = h->cR[el[t]];
+    for (int64_t m = 0; m < nr; m++)
+      for (int q = 0; q < 4; q++) h64[oRhoV + (t * nr + m) * kRhoRec + q] = R[m * 7 + 3 + q];
+  }
+  for (int a = 0; a < 3; a++)
+    for (int b2 = 0; b2 < 3; b2++) {
+      if (el[a] < 0 || el[b2] < 0) continue;
+      const int ea = std::max(el[a], el[b2]), eb = std::min(el[a], el[b2]);
+      const std::vector<double>& Ra = h->cR[el[a]];
+      const std::vector<double>& Rb = h->cR[el[b2]];
+      const std::vector<double>& Z = h->cZ[ea * (ea + 1) / 2 + eb];
+      for (int64_t m = 0; m < nr; m++) {
+        double* r = &h64[oPair + ((a * 3 + b2) * nr + m) * kPairRec];
+        for (int q = 0; q < 3; q++) {
+          r[q] = Ra[m * 7 + q];
+          r[3 + q] = Rb[m * 7 + q];
+        }
+        for (int q = 0; q < 7; q++) r[6 + q] = Z[m * 7 + q];
+      }
+    }
   std::vector<float> h32(h64.size());
   for (size_t i = 0; i < h64.size(); i++) h32[i] = static_cast<float>(h64[i]);
   if (!h->tab64) {
@@ -363,6 +416,8 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
   a64.rdrho = 1.0 / sf.drho;
   a64.rdr = 1.0 / sf.dr;
   a64.rhomax = (sf.nrho - 1) * sf.drho;
+  a64.rhoV = h->tab64 + oRhoV;
+  a64.pairF = h->tab64 + oPair;
   Tables<float>& a32 = h->t32;
   a32.F = h->tab32;
   a32.rho = h->tab32 + 3 * sF;
@@ -374,6 +429,8 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
   a32.rdrho = static_cast<float>(1.0 / sf.drho);
   a32.rdr = static_cast<float>(1.0 / sf.dr);
   a32.rhomax = static_cast<float>(a64.rhomax);
+  a32.rhoV = h->tab32 + oRhoV;
+  a32.pairF = h->tab32 + oPair;
   h->have_pot = true;
   return MISA_OK;
 }

@@ -31,6 +31,15 @@ constexpr int kKnot = 8;              // values per knot record: c0 c1 c2 _ c3 c
 constexpr int8_t kVacant = -1;
 constexpr int8_t kNearInter = 0x40;
 
+// Site traversal of the resident kernels: each XCD takes a z-slab of both
+// parity planes and walks it in y-tiles of `ty` rows (z inner), so the
+// five-plane stencil window of the rows in flight stays in that XCD's L2.
+struct Walk {
+  int ty;          // rows per y-tile (divides n1)
+  int zlo[9];      // z-slab bounds per XCD: [zlo[c], zlo[c+1])
+  int64_t bpx;     // blocks per parity per XCD (max over XCDs)
+};
+
 struct Geom {
   int n0, n1, n2;  // cells per dimension of the periodic box
   int g;           // ghost cells per face
@@ -41,12 +50,20 @@ struct Geom {
   double rc2;      // (a*f)^2
   double L0, L1, L2;
   double thr;      // (0.2 a)^2, decide() threshold (atom.cpp:42)
+  Walk W;          // traversal order of the site kernels
 };
 
-struct LatOffsets {  // slot deltas of the pruned lattice-lattice neighbour list
+// Pruned lattice-lattice neighbour list: slot deltas per centre parity, and
+// the same deltas shifted by `lo` (the most negative delta) so that a buffer
+// load can take them as its non-negative scalar byte offset from a base that
+// sits `lo` slots below the array.
+struct LatOffsets {
   int n[2];
   int32_t off[2][kMaxLatOffsets];
+  int32_t lo;
+  uint32_t sh[2][kMaxLatOffsets];  // off - lo
 };
+
 
 // Spline tables: one knot record of kKnot values per knot, all tables of one
 // kind in one allocation with a fixed stride; index by species (0 Fe, 1 Cu,
@@ -60,7 +77,12 @@ struct Tables {
   int nrho, nr;
   T rdrho, rdr;   // 1/drho, 1/dr
   T rhomax;
+  // packed records for the resident kernels
+  const T* rhoV;   // [3][nr][4]: value cubic c3..c6 of rho_e
+  const T* pairF;  // [9][nr][16]: rho_ei' c0..c2, rho_ej' c0..c2, (r phi)_eiej c0..c6, 3 pad
 };
+constexpr int kPairRec = 16;
+constexpr int kRhoRec = 4;
 
 __device__ __forceinline__ bool isVacant(int8_t t) { return t < 0; }
 __device__ __forceinline__ int species(int8_t t) { return t & 3; }

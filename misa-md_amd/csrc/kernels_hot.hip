@@ -1,0 +1,354 @@
+// The two hot passes of the resident EAM step on MI355X (gfx950).
+//
+//   kRhoDf : rho_i = sum_j rho_{t_j}(r_ij) over the pruned full neighbour list,
+//            F'(rho_i) -> df (+ ghost images)   (atom::latRho + latDf + the F'
+//            forward halo, atom.cpp:102-149)
+//   kForce : f_i = sum_j (x_i - x_j) fpair_ij, then the second half kick and,
+//            in the fused mode, the next step's half kick, drift, the lattice
+//            part of atom::decide and the position halo (atom::latForce,
+//            NewtonMotion, atom::decide; simulation.cpp:164-194)
+//
+// One thread per site.  A full list (each lane owns its site's sums) replaces
+// the reference's half list with ghost partials and reverse halos: no atomics,
+// deterministic sums.  The list keeps the reference offsets whose ideal
+// separation is below rc + 0.4a (+0.01a): a lattice atom stays within 0.2a of
+// its site (decide() turns it into an inter atom otherwise), so every dropped
+// offset is beyond the cutoff for every lattice pair and the visited pair set
+// is the reference's.
+//
+// Memory access: every lane of a wavefront reads neighbour k at the same slot
+// delta, so x, type and F' are fetched with buffer loads whose per-lane
+// offset (site * size) is fixed for the whole loop and whose scalar offset is
+// the (shifted, non-negative) delta: no per-neighbour address arithmetic.
+// Neighbours are taken in groups of U: the group's position and type loads
+// are all issued before the first distance test, and the spline records of
+// the in-cutoff pairs are all issued before the first pair term, so a wave
+// keeps U gathers in flight instead of waiting on each neighbour in turn.
+// Spline records are packed per knot: 4 values (density cubic) for the rho
+// pass, 16 values (rho'_i, rho'_j, r*phi and its derivative) per ordered
+// species pair for the force pass -- one gather per pair.
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "resident.hpp"
+#include "site_common.hpp"
+
+namespace misa {
+
+namespace {
+
+using namespace site;
+
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+
+__device__ __forceinline__ Rsrc rsrcAt(uintptr_t base) {
+  return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(base), 0, 0x7fffffff, 0x00020000);
+}
+// Resource whose base sits `lo` elements (lo <= 0) below array `p`.
+template <typename T>
+__device__ __forceinline__ Rsrc rsrcBelow(const T* p, int lo) {
+  return rsrcAt(reinterpret_cast<uintptr_t>(p) + static_cast<intptr_t>(lo) * static_cast<intptr_t>(sizeof(T)));
+}
+
+__device__ __forceinline__ double ldD(Rsrc r, uint32_t v, uint32_t s) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
+}
+__device__ __forceinline__ float ldF(Rsrc r, uint32_t v, uint32_t s) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, 0));
+}
+__device__ __forceinline__ int8_t ldB(Rsrc r, uint32_t v, uint32_t s) {
+  return static_cast<int8_t>(__builtin_amdgcn_raw_buffer_load_b8(r, v, s, 0));
+}
+template <typename T>
+__device__ __forceinline__ T ldT(Rsrc r, uint32_t v, uint32_t s);
+template <>
+__device__ __forceinline__ double ldT<double>(Rsrc r, uint32_t v, uint32_t s) {
+  return ldD(r, v, s);
+}
+template <>
+__device__ __forceinline__ float ldT<float>(Rsrc r, uint32_t v, uint32_t s) {
+  return ldF(r, v, s);
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// First N values of a record at byte offset `o` of resource `r`.  The 16-byte
+// load is reinterpreted as a whole (float4 / double2): bit-casting single
+// elements of the loaded vector is miscompiled by this toolchain (only lane 0
+// is kept).
+template <typename T, int N>
+__device__ __forceinline__ void ldRec(Rsrc r, uint32_t o, T (&out)[N]) {
+  constexpr int per = 16 / sizeof(T);
+  constexpr int full = N / per;
+#pragma unroll
+  for (int q = 0; q < full; q++) {
+    const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(r, o + 16 * q, 0, 0);
+    if constexpr (sizeof(T) == 4) {
+      const float4 f = __builtin_bit_cast(float4, w);
+      out[4 * q + 0] = f.x;
+      out[4 * q + 1] = f.y;
+      out[4 * q + 2] = f.z;
+      out[4 * q + 3] = f.w;
+    } else {
+      const double2 d = __builtin_bit_cast(double2, w);
+      out[2 * q + 0] = d.x;
+      out[2 * q + 1] = d.y;
+    }
+  }
+#pragma unroll
+  for (int q = full * per; q < N; q++) out[q] = ldT<T>(r, o + q * static_cast<uint32_t>(sizeof(T)), 0);
+}
+
+__device__ __forceinline__ double rsqT(double x) { return rsqrtRefined(x); }
+__device__ __forceinline__ float rsqT(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// ---------------------------------------------------------------------------
+// pass 1: rho and F'
+// ---------------------------------------------------------------------------
+template <typename T, int ACC, int U>
+__global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T> tb, CXPtr x,
+                                                 const int8_t* __restrict__ type, T* __restrict__ df,
+                                                 double* __restrict__ rho_out) {
+  int64_t s;
+  int p, cx, cy, cz;
+  if (!siteOfThread(G, s, p, cx, cy, cz)) return;
+  const int8_t ti = type[s];
+  T dfi = T(0);
+  double rho = 0.0;
+  if (!isVacant(ti)) {
+    const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
+    const Rsrc rx0 = rsrcBelow(x.c[0], O.lo), rx1 = rsrcBelow(x.c[1], O.lo), rx2 = rsrcBelow(x.c[2], O.lo);
+    const Rsrc rt = rsrcBelow(type, O.lo);
+    const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoV));
+    const uint32_t o8 = static_cast<uint32_t>(s) * 8u, o1 = static_cast<uint32_t>(s);
+    const uint32_t* sh = O.sh[p];
+    const int nk = O.n[p];
+    Sum1<ACC> acc;
+    auto group = [&](auto UC, int k0) {
+      constexpr int V = decltype(UC)::value;
+      int8_t tj[V];
+      double xj0[V], xj1[V], xj2[V];
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        const uint32_t so = sh[k0 + u];
+        tj[u] = ldB(rt, o1, so);
+        xj0[u] = ldD(rx0, o8, so * 8u);
+        xj1[u] = ldD(rx1, o8, so * 8u);
+        xj2[u] = ldD(rx2, o8, so * 8u);
+      }
+      bool in[V];
+      T pp[V];
+      T rec[V][kRhoRec];
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        const double d0 = xi0 - xj0[u], d1 = xi1 - xj1[u], d2 = xi2 - xj2[u];
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        in[u] = !isVacant(tj[u]) && r2 < G.rc2;
+        if (in[u]) {
+          const T q2 = static_cast<T>(r2);
+          const int m = knotOf(q2 * rsqT(q2), tb.rdr, tb.nr, false, pp[u]);
+          const uint32_t off = (static_cast<uint32_t>(species(tj[u]) * tb.nr + m)) *
+                               static_cast<uint32_t>(kRhoRec * sizeof(T));
+          ldRec<T, kRhoRec>(rtab, off, rec[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < V; ++u)
+        if (in[u]) acc.add(((rec[u][0] * pp[u] + rec[u][1]) * pp[u] + rec[u][2]) * pp[u] + rec[u][3]);
+    };
+    int k = 0;
+    for (; k + U <= nk; k += U) group(std::integral_constant<int, U>{}, k);
+    for (; k < nk; ++k) group(std::integral_constant<int, 1>{}, k);
+    rho = acc.value();
+    dfi = dEmbed(tb, species(ti), static_cast<T>(rho));
+  }
+  df[s] = dfi;
+  if (rho_out) {
+    rho_out[s] = rho;
+    forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) {
+      df[q] = dfi;
+      rho_out[q] = rho;
+    });
+  } else {
+    forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) { df[q] = dfi; });
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass 2: force and velocity Verlet
+//   MODE 0: f stored, v += c f                        (secondstep)
+//   MODE 1: v += c f; v += c f; x' = x + dt v; decide  (secondstep + next firststep)
+//   MODE 2: f stored only                             (first force evaluation)
+// ---------------------------------------------------------------------------
+template <typename T, int ACC, int MODE, int U>
+__global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T> tb, CXPtr x, XPtr xn,
+                                                 const int8_t* __restrict__ type, const T* __restrict__ df,
+                                                 XPtr v, XPtr f, const uint64_t* __restrict__ id,
+                                                 VerletParams vp, StageArgs st, Ctl* ctl) {
+  int64_t s;
+  int p, cx, cy, cz;
+  if (!siteOfThread(G, s, p, cx, cy, cz)) return;
+  const int8_t ti = type[s];
+  const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
+  if (isVacant(ti)) {  // vacancy: keeps its stale position (atom.cpp:43-47)
+    if (MODE == 1) {
+      xn.c[0][s] = xi0;
+      xn.c[1][s] = xi1;
+      xn.c[2][s] = xi2;
+      forEachImage(G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
+        xn.c[0][q] = xi0 + sx * G.L0;
+        xn.c[1][q] = xi1 + sy * G.L1;
+        xn.c[2][q] = xi2 + sz * G.L2;
+      });
+    }
+    return;
+  }
+  const int ei = species(ti);
+  const T dfi = df[s];
+  const Rsrc rx0 = rsrcBelow(x.c[0], O.lo), rx1 = rsrcBelow(x.c[1], O.lo), rx2 = rsrcBelow(x.c[2], O.lo);
+  const Rsrc rt = rsrcBelow(type, O.lo);
+  const Rsrc rdf = rsrcBelow(df, O.lo);
+  const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairF));
+  const uint32_t o8 = static_cast<uint32_t>(s) * 8u, o1 = static_cast<uint32_t>(s);
+  const uint32_t oT = static_cast<uint32_t>(s) * static_cast<uint32_t>(sizeof(T));
+  const uint32_t pairBase = static_cast<uint32_t>(ei * 3) * static_cast<uint32_t>(tb.nr);
+  const uint32_t* sh = O.sh[p];
+  const int nk = O.n[p];
+  Sum3<ACC> acc;
+  auto group = [&](auto UC, int k0) {
+    constexpr int V = decltype(UC)::value;
+    int8_t tj[V];
+    double d0[V], d1[V], d2[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      const uint32_t so = sh[k0 + u];
+      tj[u] = ldB(rt, o1, so);
+      d0[u] = ldD(rx0, o8, so * 8u);
+      d1[u] = ldD(rx1, o8, so * 8u);
+      d2[u] = ldD(rx2, o8, so * 8u);
+    }
+    bool in[V];
+    T pp[V], ri[V], dfj[V];
+    T rec[V][13];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      d0[u] = xi0 - d0[u];
+      d1[u] = xi1 - d1[u];
+      d2[u] = xi2 - d2[u];
+      const double r2 = d0[u] * d0[u] + d1[u] * d1[u] + d2[u] * d2[u];
+      in[u] = !isVacant(tj[u]) && r2 < G.rc2;
+      if (in[u]) {
+        const uint32_t so = sh[k0 + u];
+        const T q2 = static_cast<T>(r2);
+        ri[u] = rsqT(q2);
+        const int m = knotOf(q2 * ri[u], tb.rdr, tb.nr, false, pp[u]);
+        const uint32_t off = (pairBase + static_cast<uint32_t>(species(tj[u]) * tb.nr + m)) *
+                             static_cast<uint32_t>(kPairRec * sizeof(T));
+        ldRec<T, 13>(rtab, off, rec[u]);
+        dfj[u] = ldT<T>(rdf, oT, so * static_cast<uint32_t>(sizeof(T)));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      if (in[u]) {
+        const T q = pp[u], rinv = ri[u];
+        const T rhoip = (rec[u][0] * q + rec[u][1]) * q + rec[u][2];
+        const T rhojp = (rec[u][3] * q + rec[u][4]) * q + rec[u][5];
+        const T z2p = (rec[u][6] * q + rec[u][7]) * q + rec[u][8];
+        const T z2 = ((rec[u][9] * q + rec[u][10]) * q + rec[u][11]) * q + rec[u][12];
+        const T phi = z2 * rinv;
+        const T phip = z2p * rinv - phi * rinv;
+        const T psip = dfi * rhojp + dfj[u] * rhoip + phip;
+        acc.add(d0[u], d1[u], d2[u], -psip * rinv);
+      }
+    }
+  };
+  int k = 0;
+  for (; k + U <= nk; k += U) group(std::integral_constant<int, U>{}, k);
+  for (; k < nk; ++k) group(std::integral_constant<int, 1>{}, k);
+  double f0, f1, f2;
+  sum3Value<ACC>(acc, f0, f1, f2);
+  const double c = vp.c[ei];
+  double v0 = v.c[0][s] + c * f0;
+  double v1 = v.c[1][s] + c * f1;
+  double v2 = v.c[2][s] + c * f2;
+  if (MODE == 2) {
+    f.c[0][s] = f0;
+    f.c[1][s] = f1;
+    f.c[2][s] = f2;
+    return;
+  }
+  if (MODE == 0) {
+    f.c[0][s] = f0;
+    f.c[1][s] = f1;
+    f.c[2][s] = f2;
+    v.c[0][s] = v0;
+    v.c[1][s] = v1;
+    v.c[2][s] = v2;
+    return;
+  }
+  v0 = v0 + c * f0;
+  v1 = v1 + c * f1;
+  v2 = v2 + c * f2;
+  const double y0 = xi0 + vp.dt * v0;
+  const double y1 = xi1 + vp.dt * v1;
+  const double y2 = xi2 + vp.dt * v2;
+  if (leftSite(G, p, cx, cy, cz, y0, y1, y2)) {
+    stageInter(st, ctl, refIndex(G, p, cx, cy, cz), s, id[s], ti, y0, y1, y2, v0, v1, v2);
+    v0 = v1 = v2 = 0.0;
+  }
+  v.c[0][s] = v0;
+  v.c[1][s] = v1;
+  v.c[2][s] = v2;
+  xn.c[0][s] = y0;
+  xn.c[1][s] = y1;
+  xn.c[2][s] = y2;
+  forEachImage(G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
+    xn.c[0][q] = y0 + sx * G.L0;
+    xn.c[1][q] = y1 + sy * G.L1;
+    xn.c[2][q] = y2 + sz * G.L2;
+  });
+}
+
+constexpr int kUFloat = 4;
+constexpr int kUDouble = 2;
+
+}  // namespace
+
+void launchRhoDf(const DevState& s, const LatOffsets& o, const Tables<double>& t64, const Tables<float>& t32,
+                 bool mixed, int acc, bool store_rho, hipStream_t st) {
+  double* rho = store_rho ? s.rho : nullptr;
+  const unsigned grid = siteGrid(s.G);
+  if (!mixed)
+    kRhoDf<double, 0, 4><<<grid, kBlock, 0, st>>>(s.G, o, t64, cxp(s.x[s.cur]), s.type, s.df64, rho);
+  else if (acc == 0)
+    kRhoDf<float, 0, kUFloat><<<grid, kBlock, 0, st>>>(s.G, o, t32, cxp(s.x[s.cur]), s.type, s.df32, rho);
+  else
+    kRhoDf<float, 1, kUFloat><<<grid, kBlock, 0, st>>>(s.G, o, t32, cxp(s.x[s.cur]), s.type, s.df32, rho);
+}
+
+void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t64, const Tables<float>& t32,
+                 bool mixed, int acc, int mode, const VerletParams& vp, hipStream_t st) {
+  const unsigned grid = siteGrid(s.G);
+  const CXPtr x = cxp(s.x[s.cur]);
+  const XPtr xn = xp(s.x[1 - s.cur]);
+  const StageArgs sa = stageOf(s);
+#define MISA_FORCE(T, A, M, U, TB, DF) \
+  kForce<T, A, M, U><<<grid, kBlock, 0, st>>>(s.G, o, TB, x, xn, s.type, DF, xp(s.v), xp(s.f), s.id, vp, sa, s.ctl)
+  if (!mixed) {
+    if (mode == 0) MISA_FORCE(double, 0, 0, kUDouble, t64, s.df64);
+    else if (mode == 1) MISA_FORCE(double, 0, 1, kUDouble, t64, s.df64);
+    else MISA_FORCE(double, 0, 2, kUDouble, t64, s.df64);
+  } else if (acc == 0) {
+    if (mode == 0) MISA_FORCE(float, 0, 0, kUFloat, t32, s.df32);
+    else if (mode == 1) MISA_FORCE(float, 0, 1, kUFloat, t32, s.df32);
+    else MISA_FORCE(float, 0, 2, kUFloat, t32, s.df32);
+  } else {
+    if (mode == 0) MISA_FORCE(float, 1, 0, kUFloat, t32, s.df32);
+    else if (mode == 1) MISA_FORCE(float, 1, 1, kUFloat, t32, s.df32);
+    else MISA_FORCE(float, 1, 2, kUFloat, t32, s.df32);
+  }
+#undef MISA_FORCE
+}
+
+}  // namespace misa

@@ -26,321 +26,13 @@
 
 #include "device_common.hpp"
 #include "resident.hpp"
+#include "site_common.hpp"
 
 namespace misa {
 
 namespace {
 
-constexpr int kBlock = 256;
-
-struct XPtr {
-  double* c[3];
-};
-struct CXPtr {
-  const double* c[3];
-};
-
-__device__ __forceinline__ int64_t slotOf(const Geom& G, int p, int cx, int cy, int cz) {
-  return p * G.plane + (static_cast<int64_t>(cz + G.g) * G.NY + (cy + G.g)) * G.NX + (cx + G.g);
-}
-
-// Hardware block -> (parity, site) with the XCD-aware slab order.
-__device__ __forceinline__ bool siteOfThread(const Geom& G, int64_t& s, int& p, int& cx, int& cy, int& cz) {
-  const int64_t B = (G.nsub + kBlock - 1) / kBlock;  // blocks per parity
-  const int64_t C = (B + 7) / 8;                      // blocks per parity per XCD
-  const int64_t b = blockIdx.x;
-  const int64_t k = b >> 3;
-  p = static_cast<int>(k & 1);
-  const int64_t blk = (b & 7) * C + (k >> 1);
-  if (blk >= B) return false;
-  const int64_t t = blk * kBlock + threadIdx.x;
-  if (t >= G.nsub) return false;
-  const uint32_t tt = static_cast<uint32_t>(t);
-  const uint32_t n0 = static_cast<uint32_t>(G.n0), n1 = static_cast<uint32_t>(G.n1);
-  cx = static_cast<int>(tt % n0);
-  const uint32_t r = tt / n0;
-  cy = static_cast<int>(r % n1);
-  cz = static_cast<int>(r / n1);
-  s = slotOf(G, p, cx, cy, cz);
-  return true;
-}
-
-inline unsigned siteGrid(const Geom& G) {
-  const int64_t B = (G.nsub + kBlock - 1) / kBlock;
-  const int64_t C = (B + 7) / 8;
-  return static_cast<unsigned>(16 * C);
-}
-
-__device__ __forceinline__ bool nearFace(const Geom& G, int cx, int cy, int cz) {
-  return cx < G.g || cx >= G.n0 - G.g || cy < G.g || cy >= G.n1 - G.g || cz < G.g || cz >= G.n2 - G.g;
-}
-
-// Calls fn(slot, sx, sy, sz) for every periodic image of sub-box cell
-// (cx, cy, cz) that lies in the ghost layer; (sx, sy, sz) in {-1, 0, 1} is the
-// image shift in box lengths (exchangeAtom: x_ghost = x + offset,
-// lat_particle_packer.cpp:22-32,163-194).
-template <typename Fn>
-__device__ __forceinline__ void forEachImage(const Geom& G, int p, int cx, int cy, int cz, Fn fn) {
-  if (!nearFace(G, cx, cy, cz)) return;
-#pragma unroll
-  for (int dz = -1; dz <= 1; dz++) {
-    const int iz = cz + dz * G.n2;
-    if (iz < -G.g || iz >= G.n2 + G.g) continue;
-#pragma unroll
-    for (int dy = -1; dy <= 1; dy++) {
-      const int iy = cy + dy * G.n1;
-      if (iy < -G.g || iy >= G.n1 + G.g) continue;
-#pragma unroll
-      for (int dx = -1; dx <= 1; dx++) {
-        if (dx == 0 && dy == 0 && dz == 0) continue;
-        const int ix = cx + dx * G.n0;
-        if (ix < -G.g || ix >= G.n0 + G.g) continue;
-        fn(slotOf(G, p, ix, iy, iz), dx, dy, dz);
-      }
-    }
-  }
-}
-
-// atom::decide lattice test (atom.cpp:33-49) with every product and sum
-// rounded separately, as the reference computes it.
-__device__ __noinline__ bool leftSite(const Geom& G, int p, int cx, int cy, int cz, double x0, double x1,
-                                      double x2) {
-#pragma clang fp contract(off)
-  const double i = static_cast<double>(2 * cx + p);
-  const double xt = i * 0.5 * G.a;
-  const double yt = (static_cast<double>(cy) + p * 0.5) * G.a;
-  const double zt = (static_cast<double>(cz) + p * 0.5) * G.a;
-  double dist = (x0 - xt) * (x0 - xt);
-  dist += (x1 - yt) * (x1 - yt);
-  dist += (x2 - zt) * (x2 - zt);
-  return dist > G.thr;
-}
-
-// Reference linear index of sub-box cell (cx, cy, cz) of parity p.
-__device__ __forceinline__ int64_t refIndex(const Geom& G, int p, int cx, int cy, int cz) {
-  const int64_t sx = 2 * static_cast<int64_t>(G.NX);
-  return (static_cast<int64_t>(cz + G.g) * G.NY + (cy + G.g)) * sx + (2 * (cx + G.g) + p);
-}
-
-struct StageArgs {
-  int64_t cap;
-  int64_t* key;
-  int64_t* slot;
-  uint64_t* id;
-  int8_t* type;
-  double* x[3];
-  double* v[3];
-};
-
-__device__ __forceinline__ void stageInter(const StageArgs& st, Ctl* ctl, int64_t key, int64_t slot, uint64_t id,
-                                           int8_t type, double x0, double x1, double x2, double v0, double v1,
-                                           double v2) {
-  const unsigned long long k = atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->n_stage), 1ull);
-  if (static_cast<int64_t>(k) >= st.cap) {
-    atomicOr(reinterpret_cast<unsigned long long*>(&ctl->err_flags), 1ull);
-    return;
-  }
-  st.key[k] = key;
-  st.slot[k] = slot;
-  st.id[k] = id;
-  st.type[k] = type;
-  st.x[0][k] = x0;
-  st.x[1][k] = x1;
-  st.x[2][k] = x2;
-  st.v[0][k] = v0;
-  st.v[1][k] = v1;
-  st.v[2][k] = v2;
-}
-
-// Per-site sums: FP64 accumulators (ACC 0) or FP32 with Kahan compensation
-// (ACC 1).
-template <int ACC>
-struct Sum1;
-template <>
-struct Sum1<0> {
-  double s = 0;
-  __device__ __forceinline__ void add(double v) { s += v; }
-  __device__ __forceinline__ double value() const { return s; }
-};
-template <>
-struct Sum1<1> {
-  float s = 0, c = 0;
-  __device__ __forceinline__ void add(float v) {
-    const float y = v - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-  __device__ __forceinline__ double value() const { return static_cast<double>(s); }
-};
-
-template <int ACC>
-struct Sum3;
-template <>
-struct Sum3<0> {
-  double f0 = 0, f1 = 0, f2 = 0;
-  template <typename T>
-  __device__ __forceinline__ void add(double d0, double d1, double d2, T fp) {
-    const double q = static_cast<double>(fp);
-    f0 = fma(d0, q, f0);
-    f1 = fma(d1, q, f1);
-    f2 = fma(d2, q, f2);
-  }
-};
-template <>
-struct Sum3<1> {
-  Sum1<1> a, b, c;
-  double f0 = 0, f1 = 0, f2 = 0;
-  __device__ __forceinline__ void add(double d0, double d1, double d2, float fp) {
-    a.add(static_cast<float>(d0) * fp);
-    b.add(static_cast<float>(d1) * fp);
-    c.add(static_cast<float>(d2) * fp);
-  }
-};
-template <int ACC>
-__device__ __forceinline__ void sum3Value(const Sum3<ACC>& s, double& f0, double& f1, double& f2);
-template <>
-__device__ __forceinline__ void sum3Value<0>(const Sum3<0>& s, double& f0, double& f1, double& f2) {
-  f0 = s.f0;
-  f1 = s.f1;
-  f2 = s.f2;
-}
-template <>
-__device__ __forceinline__ void sum3Value<1>(const Sum3<1>& s, double& f0, double& f1, double& f2) {
-  f0 = s.a.value();
-  f1 = s.b.value();
-  f2 = s.c.value();
-}
-
-// ---------------------------------------------------------------------------
-// pass 1: rho and F'
-// ---------------------------------------------------------------------------
-template <typename T, int ACC>
-__global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T> tb, CXPtr x,
-                                                 const int8_t* __restrict__ type, T* __restrict__ df,
-                                                 double* __restrict__ rho_out) {
-  int64_t s;
-  int p, cx, cy, cz;
-  if (!siteOfThread(G, s, p, cx, cy, cz)) return;
-  const int8_t ti = type[s];
-  T dfi = T(0);
-  double rho = 0.0;
-  if (!isVacant(ti)) {
-    const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
-    Sum1<ACC> acc;
-    const int nk = O.n[p];
-    for (int k = 0; k < nk; ++k) {
-      const int64_t j = s + O.off[p][k];
-      const int8_t tj = type[j];
-      const double d0 = xi0 - x.c[0][j];
-      const double d1 = xi1 - x.c[1][j];
-      const double d2 = xi2 - x.c[2][j];
-      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-      if (!isVacant(tj) && r2 < G.rc2) acc.add(rhoPair(tb, species(tj), r2));
-    }
-    rho = acc.value();
-    dfi = dEmbed(tb, species(ti), static_cast<T>(rho));
-  }
-  df[s] = dfi;
-  if (rho_out) {
-    rho_out[s] = rho;
-    forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) {
-      df[q] = dfi;
-      rho_out[q] = rho;
-    });
-  } else {
-    forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) { df[q] = dfi; });
-  }
-}
-
-// ---------------------------------------------------------------------------
-// pass 2: force and velocity Verlet
-//   MODE 0: f stored, v += c f                        (secondstep)
-//   MODE 1: v += c f; v += c f; x' = x + dt v; decide  (secondstep + next firststep)
-//   MODE 2: f stored only                             (first force evaluation)
-// ---------------------------------------------------------------------------
-template <typename T, int ACC, int MODE>
-__global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T> tb, CXPtr x, XPtr xn,
-                                                 const int8_t* __restrict__ type, const T* __restrict__ df,
-                                                 XPtr v, XPtr f, const uint64_t* __restrict__ id,
-                                                 VerletParams vp, StageArgs st, Ctl* ctl) {
-  int64_t s;
-  int p, cx, cy, cz;
-  if (!siteOfThread(G, s, p, cx, cy, cz)) return;
-  const int8_t ti = type[s];
-  const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
-  if (isVacant(ti)) {  // vacancy: keeps its stale position (atom.cpp:43-47)
-    if (MODE == 1) {
-      xn.c[0][s] = xi0;
-      xn.c[1][s] = xi1;
-      xn.c[2][s] = xi2;
-      forEachImage(G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
-        xn.c[0][q] = xi0 + sx * G.L0;
-        xn.c[1][q] = xi1 + sy * G.L1;
-        xn.c[2][q] = xi2 + sz * G.L2;
-      });
-    }
-    return;
-  }
-  const int ei = species(ti);
-  const T dfi = df[s];
-  Sum3<ACC> acc;
-  const int nk = O.n[p];
-  for (int k = 0; k < nk; ++k) {
-    const int64_t j = s + O.off[p][k];
-    const int8_t tj = type[j];
-    const double d0 = xi0 - x.c[0][j];
-    const double d1 = xi1 - x.c[1][j];
-    const double d2 = xi2 - x.c[2][j];
-    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-    if (!isVacant(tj) && r2 < G.rc2) {
-      const T fpair = pairForce(tb, ei, species(tj), r2, dfi, df[j]);
-      acc.add(d0, d1, d2, fpair);
-    }
-  }
-  double f0, f1, f2;
-  sum3Value<ACC>(acc, f0, f1, f2);
-  const double c = vp.c[ei];
-  double v0 = v.c[0][s] + c * f0;
-  double v1 = v.c[1][s] + c * f1;
-  double v2 = v.c[2][s] + c * f2;
-  if (MODE == 2) {
-    f.c[0][s] = f0;
-    f.c[1][s] = f1;
-    f.c[2][s] = f2;
-    return;
-  }
-  if (MODE == 0) {
-    f.c[0][s] = f0;
-    f.c[1][s] = f1;
-    f.c[2][s] = f2;
-    v.c[0][s] = v0;
-    v.c[1][s] = v1;
-    v.c[2][s] = v2;
-    return;
-  }
-  v0 = v0 + c * f0;
-  v1 = v1 + c * f1;
-  v2 = v2 + c * f2;
-  const double y0 = xi0 + vp.dt * v0;
-  const double y1 = xi1 + vp.dt * v1;
-  const double y2 = xi2 + vp.dt * v2;
-  if (leftSite(G, p, cx, cy, cz, y0, y1, y2)) {
-    stageInter(st, ctl, refIndex(G, p, cx, cy, cz), s, id[s], ti, y0, y1, y2, v0, v1, v2);
-    v0 = v1 = v2 = 0.0;
-  }
-  v.c[0][s] = v0;
-  v.c[1][s] = v1;
-  v.c[2][s] = v2;
-  xn.c[0][s] = y0;
-  xn.c[1][s] = y1;
-  xn.c[2][s] = y2;
-  forEachImage(G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
-    xn.c[0][q] = y0 + sx * G.L0;
-    xn.c[1][q] = y1 + sy * G.L1;
-    xn.c[2][q] = y2 + sz * G.L2;
-  });
-}
+using namespace site;
 
 // firststep from stored forces + lattice decide + position halo.
 __global__ __launch_bounds__(kBlock) void kFirst(Geom G, CXPtr x, XPtr xn, const int8_t* __restrict__ type,
@@ -565,23 +257,6 @@ __global__ void kAddV(XPtr v, int64_t s, double a, double b, double c) {
   v.c[2][s] += c;
 }
 
-XPtr xp(double* const (&a)[3]) { return XPtr{{a[0], a[1], a[2]}}; }
-CXPtr cxp(double* const (&a)[3]) { return CXPtr{{a[0], a[1], a[2]}}; }
-
-StageArgs stageOf(const DevState& s) {
-  StageArgs a;
-  a.cap = s.in.cap;
-  a.key = s.in.stage_key;
-  a.slot = s.in.stage_slot;
-  a.id = s.in.stage_id;
-  a.type = s.in.stage_type;
-  for (int k = 0; k < 3; k++) {
-    a.x[k] = s.in.stage_x[k];
-    a.v[k] = s.in.stage_v[k];
-  }
-  return a;
-}
-
 }  // namespace
 
 void launchUpload(const DevState& s, const AtomElement* aos_dev, int64_t sx, int64_t sy, hipStream_t st) {
@@ -597,42 +272,6 @@ void launchDownload(const DevState& s, AtomElement* aos_dev, int64_t sx, int64_t
   const unsigned grid = static_cast<unsigned>((2 * s.G.plane + kBlock - 1) / kBlock);
   kDownload<<<grid, kBlock, 0, st>>>(s.G, aos_dev, sx, sy, cxp(s.x[s.cur]), cxp(s.v), cxp(s.f), s.type, s.id,
                                      s.rho, mixed ? nullptr : s.df64, s.df32);
-}
-
-void launchRhoDf(const DevState& s, const LatOffsets& o, const Tables<double>& t64, const Tables<float>& t32,
-                 bool mixed, int acc, bool store_rho, hipStream_t st) {
-  double* rho = store_rho ? s.rho : nullptr;
-  const unsigned grid = siteGrid(s.G);
-  if (!mixed)
-    kRhoDf<double, 0><<<grid, kBlock, 0, st>>>(s.G, o, t64, cxp(s.x[s.cur]), s.type, s.df64, rho);
-  else if (acc == 0)
-    kRhoDf<float, 0><<<grid, kBlock, 0, st>>>(s.G, o, t32, cxp(s.x[s.cur]), s.type, s.df32, rho);
-  else
-    kRhoDf<float, 1><<<grid, kBlock, 0, st>>>(s.G, o, t32, cxp(s.x[s.cur]), s.type, s.df32, rho);
-}
-
-void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t64, const Tables<float>& t32,
-                 bool mixed, int acc, int mode, const VerletParams& vp, hipStream_t st) {
-  const unsigned grid = siteGrid(s.G);
-  const CXPtr x = cxp(s.x[s.cur]);
-  const XPtr xn = xp(s.x[1 - s.cur]);
-  const StageArgs sa = stageOf(s);
-#define MISA_FORCE(T, A, M, TB, DF) \
-  kForce<T, A, M><<<grid, kBlock, 0, st>>>(s.G, o, TB, x, xn, s.type, DF, xp(s.v), xp(s.f), s.id, vp, sa, s.ctl)
-  if (!mixed) {
-    if (mode == 0) MISA_FORCE(double, 0, 0, t64, s.df64);
-    else if (mode == 1) MISA_FORCE(double, 0, 1, t64, s.df64);
-    else MISA_FORCE(double, 0, 2, t64, s.df64);
-  } else if (acc == 0) {
-    if (mode == 0) MISA_FORCE(float, 0, 0, t32, s.df32);
-    else if (mode == 1) MISA_FORCE(float, 0, 1, t32, s.df32);
-    else MISA_FORCE(float, 0, 2, t32, s.df32);
-  } else {
-    if (mode == 0) MISA_FORCE(float, 1, 0, t32, s.df32);
-    else if (mode == 1) MISA_FORCE(float, 1, 1, t32, s.df32);
-    else MISA_FORCE(float, 1, 2, t32, s.df32);
-  }
-#undef MISA_FORCE
 }
 
 void launchFirst(const DevState& s, const VerletParams& vp, hipStream_t st) {

@@ -1,0 +1,211 @@
+// Device helpers shared by the resident-mode site kernels: site traversal,
+// periodic images, the lattice ownership test, inter-atom staging and the
+// per-site accumulators.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "resident.hpp"
+
+namespace misa {
+namespace site {
+
+constexpr int kBlock = 256;
+
+struct XPtr {
+  double* c[3];
+};
+struct CXPtr {
+  const double* c[3];
+};
+
+__device__ __forceinline__ int64_t slotOf(const Geom& G, int p, int cx, int cy, int cz) {
+  return p * G.plane + (static_cast<int64_t>(cz + G.g) * G.NY + (cy + G.g)) * G.NX + (cx + G.g);
+}
+
+// Hardware block -> (parity, site).  Block b runs on XCD c = b % 8 (the
+// dispatcher deals workgroups to the XCDs round-robin); the k-th block of XCD
+// c takes parity k & 1 and local block k >> 1 of that XCD's z-slab, whose
+// sites are ordered (y-tile, z, y within tile, x).
+__device__ __forceinline__ bool siteOfThread(const Geom& G, int64_t& s, int& p, int& cx, int& cy, int& cz) {
+  const uint32_t b = blockIdx.x;
+  const int c = static_cast<int>(b & 7);
+  const uint32_t k = b >> 3;
+  p = static_cast<int>(k & 1);
+  const int z0 = G.W.zlo[c], nzc = G.W.zlo[c + 1] - z0;
+  const uint32_t t = (k >> 1) * kBlock + threadIdx.x;
+  const uint32_t n0 = static_cast<uint32_t>(G.n0), ty = static_cast<uint32_t>(G.W.ty);
+  if (nzc <= 0 || t >= static_cast<uint32_t>(nzc) * static_cast<uint32_t>(G.n1) * n0) return false;
+  cx = static_cast<int>(t % n0);
+  uint32_t r = t / n0;
+  const int yy = static_cast<int>(r % ty);
+  r /= ty;
+  cz = z0 + static_cast<int>(r % static_cast<uint32_t>(nzc));
+  cy = static_cast<int>(r / static_cast<uint32_t>(nzc)) * G.W.ty + yy;
+  s = slotOf(G, p, cx, cy, cz);
+  return true;
+}
+
+inline unsigned siteGrid(const Geom& G) { return static_cast<unsigned>(16 * G.W.bpx); }
+
+__device__ __forceinline__ bool nearFace(const Geom& G, int cx, int cy, int cz) {
+  return cx < G.g || cx >= G.n0 - G.g || cy < G.g || cy >= G.n1 - G.g || cz < G.g || cz >= G.n2 - G.g;
+}
+
+// Calls fn(slot, sx, sy, sz) for every periodic image of sub-box cell
+// (cx, cy, cz) that lies in the ghost layer; (sx, sy, sz) in {-1, 0, 1} is the
+// image shift in box lengths (exchangeAtom: x_ghost = x + offset,
+// lat_particle_packer.cpp:22-32,163-194).
+template <typename Fn>
+__device__ __forceinline__ void forEachImage(const Geom& G, int p, int cx, int cy, int cz, Fn fn) {
+  if (!nearFace(G, cx, cy, cz)) return;
+#pragma unroll
+  for (int dz = -1; dz <= 1; dz++) {
+    const int iz = cz + dz * G.n2;
+    if (iz < -G.g || iz >= G.n2 + G.g) continue;
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++) {
+      const int iy = cy + dy * G.n1;
+      if (iy < -G.g || iy >= G.n1 + G.g) continue;
+#pragma unroll
+      for (int dx = -1; dx <= 1; dx++) {
+        if (dx == 0 && dy == 0 && dz == 0) continue;
+        const int ix = cx + dx * G.n0;
+        if (ix < -G.g || ix >= G.n0 + G.g) continue;
+        fn(slotOf(G, p, ix, iy, iz), dx, dy, dz);
+      }
+    }
+  }
+}
+
+// atom::decide lattice test (atom.cpp:33-49) with every product and sum
+// rounded separately, as the reference computes it.
+__device__ __forceinline__ bool leftSite(const Geom& G, int p, int cx, int cy, int cz, double x0, double x1,
+                                      double x2) {
+#pragma clang fp contract(off)
+  const double i = static_cast<double>(2 * cx + p);
+  const double xt = i * 0.5 * G.a;
+  const double yt = (static_cast<double>(cy) + p * 0.5) * G.a;
+  const double zt = (static_cast<double>(cz) + p * 0.5) * G.a;
+  double dist = (x0 - xt) * (x0 - xt);
+  dist += (x1 - yt) * (x1 - yt);
+  dist += (x2 - zt) * (x2 - zt);
+  return dist > G.thr;
+}
+
+// Reference linear index of sub-box cell (cx, cy, cz) of parity p.
+__device__ __forceinline__ int64_t refIndex(const Geom& G, int p, int cx, int cy, int cz) {
+  const int64_t sx = 2 * static_cast<int64_t>(G.NX);
+  return (static_cast<int64_t>(cz + G.g) * G.NY + (cy + G.g)) * sx + (2 * (cx + G.g) + p);
+}
+
+struct StageArgs {
+  int64_t cap;
+  int64_t* key;
+  int64_t* slot;
+  uint64_t* id;
+  int8_t* type;
+  double* x[3];
+  double* v[3];
+};
+
+__device__ __forceinline__ void stageInter(const StageArgs& st, Ctl* ctl, int64_t key, int64_t slot, uint64_t id,
+                                           int8_t type, double x0, double x1, double x2, double v0, double v1,
+                                           double v2) {
+  const unsigned long long k = atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->n_stage), 1ull);
+  if (static_cast<int64_t>(k) >= st.cap) {
+    atomicOr(reinterpret_cast<unsigned long long*>(&ctl->err_flags), 1ull);
+    return;
+  }
+  st.key[k] = key;
+  st.slot[k] = slot;
+  st.id[k] = id;
+  st.type[k] = type;
+  st.x[0][k] = x0;
+  st.x[1][k] = x1;
+  st.x[2][k] = x2;
+  st.v[0][k] = v0;
+  st.v[1][k] = v1;
+  st.v[2][k] = v2;
+}
+
+// Per-site sums: FP64 accumulators (ACC 0) or FP32 with Kahan compensation
+// (ACC 1).
+template <int ACC>
+struct Sum1;
+template <>
+struct Sum1<0> {
+  double s = 0;
+  __device__ __forceinline__ void add(double v) { s += v; }
+  __device__ __forceinline__ double value() const { return s; }
+};
+template <>
+struct Sum1<1> {
+  float s = 0, c = 0;
+  __device__ __forceinline__ void add(float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return static_cast<double>(s); }
+};
+
+template <int ACC>
+struct Sum3;
+template <>
+struct Sum3<0> {
+  double f0 = 0, f1 = 0, f2 = 0;
+  template <typename T>
+  __device__ __forceinline__ void add(double d0, double d1, double d2, T fp) {
+    const double q = static_cast<double>(fp);
+    f0 = fma(d0, q, f0);
+    f1 = fma(d1, q, f1);
+    f2 = fma(d2, q, f2);
+  }
+};
+template <>
+struct Sum3<1> {
+  Sum1<1> a, b, c;
+  double f0 = 0, f1 = 0, f2 = 0;
+  __device__ __forceinline__ void add(double d0, double d1, double d2, float fp) {
+    a.add(static_cast<float>(d0) * fp);
+    b.add(static_cast<float>(d1) * fp);
+    c.add(static_cast<float>(d2) * fp);
+  }
+};
+template <int ACC>
+__device__ __forceinline__ void sum3Value(const Sum3<ACC>& s, double& f0, double& f1, double& f2);
+template <>
+__device__ __forceinline__ void sum3Value<0>(const Sum3<0>& s, double& f0, double& f1, double& f2) {
+  f0 = s.f0;
+  f1 = s.f1;
+  f2 = s.f2;
+}
+template <>
+__device__ __forceinline__ void sum3Value<1>(const Sum3<1>& s, double& f0, double& f1, double& f2) {
+  f0 = s.a.value();
+  f1 = s.b.value();
+  f2 = s.c.value();
+}
+
+inline XPtr xp(double* const (&a)[3]) { return XPtr{{a[0], a[1], a[2]}}; }
+inline CXPtr cxp(double* const (&a)[3]) { return CXPtr{{a[0], a[1], a[2]}}; }
+
+inline StageArgs stageOf(const DevState& s) {
+  StageArgs a;
+  a.cap = s.in.cap;
+  a.key = s.in.stage_key;
+  a.slot = s.in.stage_slot;
+  a.id = s.in.stage_id;
+  a.type = s.in.stage_type;
+  for (int k = 0; k < 3; k++) {
+    a.x[k] = s.in.stage_x[k];
+    a.v[k] = s.in.stage_v[k];
+  }
+  return a;
+}
+
+
+}  // namespace site
+}  // namespace misa
