@@ -383,16 +383,14 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
     for (int b2 = 0; b2 < 3; b2++) {
       if (el[a] < 0 || el[b2] < 0) continue;
       const int ea = std::max(el[a], el[b2]), eb = std::min(el[a], el[b2]);
-      const std::vector<double>& Ra = h->cR[el[a]];
       const std::vector<double>& Rb = h->cR[el[b2]];
       const std::vector<double>& Z = h->cZ[ea * (ea + 1) / 2 + eb];
       for (int64_t m = 0; m < nr; m++) {
         double* r = &h64[oPair + ((a * 3 + b2) * nr + m) * kPairRec];
-        for (int q = 0; q < 3; q++) {
-          r[q] = Ra[m * 7 + q];
-          r[3 + q] = Rb[m * 7 + q];
+        for (int q = 0; q < 4; q++) {
+          r[q] = Z[m * 7 + 3 + q];
+          r[4 + q] = Rb[m * 7 + 3 + q];
         }
-        for (int q = 0; q < 7; q++) r[6 + q] = Z[m * 7 + q];
       }
     }
   std::vector<float> h32(h64.size());

@@ -79,9 +79,9 @@ struct Tables {
   T rhomax;
   // packed records for the resident kernels
   const T* rhoV;   // [3][nr][4]: value cubic c3..c6 of rho_e
-  const T* pairF;  // [9][nr][16]: rho_ei' c0..c2, rho_ej' c0..c2, (r phi)_eiej c0..c6, 3 pad
+  const T* pairF;  // [9][nr][8]: value cubics of (r phi)_eiej and rho_ej (c3..c6 each)
 };
-constexpr int kPairRec = 16;
+constexpr int kPairRec = 8;
 constexpr int kRhoRec = 4;
 
 __device__ __forceinline__ bool isVacant(int8_t t) { return t < 0; }

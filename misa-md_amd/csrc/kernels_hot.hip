@@ -24,9 +24,11 @@
 // are all issued before the first distance test, and the spline records of
 // the in-cutoff pairs are all issued before the first pair term, so a wave
 // keeps U gathers in flight instead of waiting on each neighbour in turn.
-// Spline records are packed per knot: 4 values (density cubic) for the rho
-// pass, 16 values (rho'_i, rho'_j, r*phi and its derivative) per ordered
-// species pair for the force pass -- one gather per pair.
+// Spline records are packed per knot: the density value cubic (4 values) per
+// species for the rho pass, and per ordered species pair the value cubics of
+// r*phi and of the neighbour's density (8 values) for the force pass; the
+// derivatives are taken from the value cubics, and the centre's density
+// cubic is fetched separately only for unlike pairs.
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
@@ -209,6 +211,7 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
   const Rsrc rt = rsrcBelow(type, O.lo);
   const Rsrc rdf = rsrcBelow(df, O.lo);
   const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairF));
+  const Rsrc rtabR = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoV));
   const uint32_t o8 = static_cast<uint32_t>(s) * 8u, o1 = static_cast<uint32_t>(s);
   const uint32_t oT = static_cast<uint32_t>(s) * static_cast<uint32_t>(sizeof(T));
   const uint32_t pairBase = static_cast<uint32_t>(ei * 3) * static_cast<uint32_t>(tb.nr);
@@ -229,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
     }
     bool in[V];
     T pp[V], ri[V], dfj[V];
-    T rec[V][13];
+    T rec[V][kPairRec], reci[V][kRhoRec];
 #pragma unroll
     for (int u = 0; u < V; ++u) {
       d0[u] = xi0 - d0[u];
@@ -242,20 +245,33 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
         const T q2 = static_cast<T>(r2);
         ri[u] = rsqT(q2);
         const int m = knotOf(q2 * ri[u], tb.rdr, tb.nr, false, pp[u]);
-        const uint32_t off = (pairBase + static_cast<uint32_t>(species(tj[u]) * tb.nr + m)) *
+        const int ej = species(tj[u]);
+        const uint32_t off = (pairBase + static_cast<uint32_t>(ej * tb.nr + m)) *
                              static_cast<uint32_t>(kPairRec * sizeof(T));
-        ldRec<T, 13>(rtab, off, rec[u]);
+        ldRec<T, kPairRec>(rtab, off, rec[u]);
+        if (ej != ei) {  // rho_ei differs from rho_ej only for unlike pairs
+          const uint32_t offi = static_cast<uint32_t>(ei * tb.nr + m) * static_cast<uint32_t>(kRhoRec * sizeof(T));
+          ldRec<T, kRhoRec>(rtabR, offi, reci[u]);
+        } else {
+#pragma unroll
+          for (int q = 0; q < kRhoRec; q++) reci[u][q] = rec[u][4 + q];
+        }
         dfj[u] = ldT<T>(rdf, oT, so * static_cast<uint32_t>(sizeof(T)));
       }
     }
 #pragma unroll
     for (int u = 0; u < V; ++u) {
       if (in[u]) {
+        // value cubics c3..c6 in the knot-local coordinate q; derivatives
+        // (3 c3 q + 2 c4) q + c5 per knot spacing, times 1/dr
         const T q = pp[u], rinv = ri[u];
-        const T rhoip = (rec[u][0] * q + rec[u][1]) * q + rec[u][2];
-        const T rhojp = (rec[u][3] * q + rec[u][4]) * q + rec[u][5];
-        const T z2p = (rec[u][6] * q + rec[u][7]) * q + rec[u][8];
-        const T z2 = ((rec[u][9] * q + rec[u][10]) * q + rec[u][11]) * q + rec[u][12];
+        const T* z = rec[u];
+        const T* rj = rec[u] + 4;
+        const T* rI = reci[u];
+        const T z2 = ((z[0] * q + z[1]) * q + z[2]) * q + z[3];
+        const T z2p = ((T(3) * z[0] * q + T(2) * z[1]) * q + z[2]) * tb.rdr;
+        const T rhojp = ((T(3) * rj[0] * q + T(2) * rj[1]) * q + rj[2]) * tb.rdr;
+        const T rhoip = ((T(3) * rI[0] * q + T(2) * rI[1]) * q + rI[2]) * tb.rdr;
         const T phi = z2 * rinv;
         const T phip = z2p * rinv - phi * rinv;
         const T psip = dfi * rhojp + dfj[u] * rhoip + phip;
