@@ -25,6 +25,10 @@ struct misa_handle {
   bool have_offsets = false;
   LatOffsets lat{};
   FullOffsets full{};
+  FullPlaneOffsets fullPlane{};
+  void* interBlock = nullptr;  // one allocation carved into the inter-atom arrays
+  AtomElement* interAos = nullptr;
+  int64_t interAosCap = 0;
   Setfl setfl{};
   bool have_pot = false;
   std::vector<std::vector<double>> cF, cR, cZ;  // 7-coefficient tables by file element
@@ -42,6 +46,7 @@ struct misa_handle {
   std::vector<void*> allocs;
   bool profile = false;
   std::vector<hipEvent_t> events;
+  std::vector<int> event_cls;
   double timings[4] = {0, 0, 0, 0};
   int64_t launches[3] = {0, 0, 0};
 };
@@ -93,6 +98,11 @@ int buildPlaneOffsets(misa_handle* h) {
       if (std::abs(po.dcx) > b.g || std::abs(po.dy) > b.g || std::abs(po.dz) > b.g)
         return fail(h, MISA_E_ARG, "offset reaches beyond the ghost layer");
       h->full.off[p][k] = static_cast<int32_t>(L[k]);
+      {
+        const int64_t NX = b.n[0] + 2 * b.g, NY = b.n[1] + 2 * b.g, NZ = b.n[2] + 2 * b.g;
+        h->fullPlane.off[p][k] =
+            static_cast<int32_t>(po.dp * NX * NY * NZ + (static_cast<int64_t>(po.dz) * NY + po.dy) * NX + po.dcx);
+      }
       if (po.len < keep) {
         if (nl >= kMaxLatOffsets) return fail(h, MISA_E_ARG, "pruned offset list too long");
         const int64_t NX = b.n[0] + 2 * b.g, NY = b.n[1] + 2 * b.g, NZ = b.n[2] + 2 * b.g;
@@ -102,6 +112,12 @@ int buildPlaneOffsets(misa_handle* h) {
       }
     }
     h->lat.n[p] = nl;
+    h->fullPlane.n[p] = static_cast<int>(L.size());
+  }
+  if (h->fullPlane.n[0] != h->fullPlane.n[1]) return fail(h, MISA_E_ARG, "even and odd offset lists differ in length");
+  if (h->s.in.full) {
+    hipError_t e = hipMemcpy(h->s.in.full, &h->fullPlane, sizeof(FullPlaneOffsets), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(h, MISA_E_HIP, std::string("hipMemcpy offsets: ") + hipGetErrorString(e));
   }
   // non-negative form for the buffer loads: sh = off - lo, lo = most negative delta
   int32_t lo = 0;
@@ -139,9 +155,76 @@ int syncAndCheck(misa_handle* h) {
   if (c.err_flags & 1) return fail(h, MISA_E_CAPACITY, "inter-atom list capacity exceeded");
   if (c.err_flags & 2) return fail(h, MISA_E_CAPACITY, "ghost inter-atom capacity exceeded");
   if (c.err_flags & 4) return fail(h, MISA_E_DEVICE, "non-finite force");
-  if (c.err_flags & ~int64_t(7)) return fail(h, MISA_E_DEVICE, "device error flag set");
-  if (c.n_stage > 0)
-    return fail(h, MISA_E_DEVICE, "lattice atoms left their sites; the inter-atom path is not enabled in this build");
+  if (c.err_flags & 8) return fail(h, MISA_E_DEVICE, "an inter atom left the ghost-extended box");
+  if (c.err_flags & ~int64_t(15)) return fail(h, MISA_E_DEVICE, "device error flag set");
+  return MISA_OK;
+}
+
+// Inter-atom arrays for `cap` local atoms in one allocation; per-slot bucket
+// heads and recombination claims are sized by the box.
+int allocInter(misa_handle* h, int64_t cap) {
+  InterDev& in = h->s.in;
+  if (h->interBlock) {
+    hipFree(h->interBlock);
+    h->interBlock = nullptr;
+  }
+  const int64_t ecap = 4 * cap;
+  int64_t kcap = 1;
+  while (kcap < ecap) kcap <<= 1;
+  const int64_t slots = 2 * h->s.G.plane;
+  size_t bytes = 0;
+  auto need = [&](size_t n) {
+    const size_t off = bytes;
+    bytes += (n + 255) & ~size_t(255);
+    return off;
+  };
+  struct Part {
+    void** p;
+    size_t off;
+  };
+  std::vector<Part> parts;
+  auto add = [&](auto** ptr, size_t n, size_t elem) { parts.push_back({reinterpret_cast<void**>(ptr), need(n * elem)}); };
+  add(&in.id, cap, 8);
+  add(&in.type, cap, 1);
+  add(&in.id2, cap, 8);
+  add(&in.type2, cap, 1);
+  add(&in.rho, cap, 8);
+  add(&in.df64, cap, 8);
+  add(&in.df32, cap, 4);
+  add(&in.tmp, cap, 4);
+  add(&in.stage_key, cap, 8);
+  add(&in.stage_slot, cap, 8);
+  add(&in.stage_id, cap, 8);
+  add(&in.stage_type, cap, 1);
+  for (int k = 0; k < 3; k++) {
+    add(&in.x[k], cap, 8);
+    add(&in.v[k], cap, 8);
+    add(&in.f[k], cap, 8);
+    add(&in.x2[k], cap, 8);
+    add(&in.v2[k], cap, 8);
+    add(&in.stage_x[k], cap, 8);
+    add(&in.stage_v[k], cap, 8);
+    add(&in.e_x[k], ecap, 8);
+  }
+  add(&in.e_src, ecap, 4);
+  add(&in.e_slot, ecap, 8);
+  add(&in.prev_slot, ecap, 8);
+  add(&in.keys, kcap, 8);
+  add(&in.head, slots, 4);
+  add(&in.site_claim, slots, 4);
+  add(&in.full, 1, sizeof(FullPlaneOffsets));
+  char* base = nullptr;
+  hipError_t e = hipMalloc(&base, bytes);
+  if (e != hipSuccess) return fail(h, MISA_E_HIP, std::string("hipMalloc inter: ") + hipGetErrorString(e));
+  h->interBlock = base;
+  for (auto& pt : parts) *pt.p = base + pt.off;
+  in.cap = cap;
+  in.ecap = ecap;
+  in.kcap = kcap;
+  if (hipMemset(base, 0, bytes) != hipSuccess || hipMemset(in.head, 0xff, slots * 4) != hipSuccess ||
+      hipMemset(in.site_claim, 0x7f, slots * 4) != hipSuccess ||
+      hipMemcpy(in.full, &h->fullPlane, sizeof(FullPlaneOffsets), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(h, MISA_E_HIP, "initialising the inter-atom arrays failed");
   return MISA_OK;
 }
 
@@ -152,6 +235,7 @@ struct Timer {
   Timer(misa_handle* hh, int c) : h(hh), cls(c) {
     if (!h->profile) return;
     e0 = h->events.size();
+    h->event_cls.push_back(c);
     hipEvent_t a, b;
     hipEventCreate(&a);
     hipEventCreate(&b);
@@ -231,16 +315,7 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
   const int64_t blocks = 16 * G.W.bpx;
   s.npartials = 8 * blocks;
   if (rc == MISA_OK) rc |= devAlloc(h, &s.partials, static_cast<size_t>(s.npartials));
-  // inter-atom staging (default capacity)
-  s.in.cap = 65536;
-  if (rc == MISA_OK) rc |= devAlloc(h, &s.in.stage_key, s.in.cap);
-  if (rc == MISA_OK) rc |= devAlloc(h, &s.in.stage_slot, s.in.cap);
-  if (rc == MISA_OK) rc |= devAlloc(h, &s.in.stage_id, s.in.cap);
-  if (rc == MISA_OK) rc |= devAlloc(h, &s.in.stage_type, s.in.cap);
-  for (int k = 0; k < 3 && rc == MISA_OK; k++) {
-    rc |= devAlloc(h, &s.in.stage_x[k], s.in.cap);
-    rc |= devAlloc(h, &s.in.stage_v[k], s.in.cap);
-  }
+  if (rc == MISA_OK) rc |= allocInter(h, 65536);
   if (rc != MISA_OK) {
     misa_destroy(h);
     return MISA_E_HIP;
@@ -271,6 +346,8 @@ int misa_destroy(misa_handle* h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   for (void* p : h->allocs) hipFree(p);
   if (h->aos_dev) hipFree(h->aos_dev);
+  if (h->interBlock) hipFree(h->interBlock);
+  if (h->interAos) hipFree(h->interAos);
   for (auto e : h->events) hipEventDestroy(e);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -461,22 +538,24 @@ int misa_lat_df(misa_handle* h, void* atoms, double rc) { return hookCall(h, 1, 
 int misa_lat_force(misa_handle* h, void* atoms, double rc) { return hookCall(h, 2, atoms, rc); }
 
 int misa_upload_state(misa_handle* h, const void* atoms, const void* inter, int64_t n_inter) {
-  if (!h || !atoms) return MISA_E_ARG;
-  if (n_inter != 0 || inter) {
-    if (n_inter > 0) return fail(h, MISA_E_ARG, "inter-atom upload is not enabled in this build");
-  }
+  if (!h || !atoms || n_inter < 0 || (n_inter > 0 && !inter)) return MISA_E_ARG;
+  if (n_inter > h->s.in.cap) return fail(h, MISA_E_CAPACITY, "inter list larger than the inter-atom capacity");
   HIPCHECK(h, hipSetDevice(h->device));
   const AtomElement* a = static_cast<const AtomElement*>(atoms);
   const Box& b = h->box;
   // species present in the potential?
+  auto checkType = [&](int t) -> bool {
+    if (t > 2 || t < -1) return false;
+    return !(t >= 0 && h->have_pot && !h->species_present[t]);
+  };
   for (int64_t k = 0; k < b.n[2]; k++)
     for (int64_t j = 0; j < b.n[1]; j++)
-      for (int64_t i = 0; i < 2 * b.n[0]; i++) {
-        const int t = a[b.refIndex(i + 2 * b.g, j + b.g, k + b.g)].type;
-        if (t > 2) return fail(h, MISA_E_ARG, "unknown atom type");
-        if (t >= 0 && h->have_pot && !h->species_present[t])
-          return fail(h, MISA_E_ARG, "atom species missing from the potential file");
-      }
+      for (int64_t i = 0; i < 2 * b.n[0]; i++)
+        if (!checkType(a[b.refIndex(i + 2 * b.g, j + b.g, k + b.g)].type))
+          return fail(h, MISA_E_ARG, "atom type unknown or missing from the potential file");
+  const AtomElement* ia = static_cast<const AtomElement*>(inter);
+  for (int64_t i = 0; i < n_inter; i++)
+    if (ia[i].type < 0 || !checkType(ia[i].type)) return fail(h, MISA_E_ARG, "bad inter-atom type");
   const size_t bytes = static_cast<size_t>(b.slots()) * sizeof(AtomElement);
   if (!h->aos_dev) HIPCHECK(h, hipMalloc(&h->aos_dev, bytes));
   HIPCHECK(h, hipMemcpyAsync(h->aos_dev, atoms, bytes, hipMemcpyHostToDevice, h->stream));
@@ -485,6 +564,18 @@ int misa_upload_state(misa_handle* h, const void* atoms, const void* inter, int6
   launchHaloAll(h->s, h->s.cur, h->stream);
   HIPCHECK(h, hipGetLastError());
   HIPCHECK(h, hipMemsetAsync(h->s.ctl, 0, sizeof(Ctl), h->stream));
+  HIPCHECK(h, hipMemsetAsync(h->s.in.head, 0xff, static_cast<size_t>(2 * h->s.G.plane) * 4, h->stream));
+  if (n_inter > 0) {
+    if (h->interAosCap < n_inter) {
+      if (h->interAos) HIPCHECK(h, hipFree(h->interAos));
+      h->interAos = nullptr;
+      HIPCHECK(h, hipMalloc(&h->interAos, static_cast<size_t>(h->s.in.cap) * sizeof(AtomElement)));
+      h->interAosCap = h->s.in.cap;
+    }
+    HIPCHECK(h, hipMemcpyAsync(h->interAos, inter, n_inter * sizeof(AtomElement), hipMemcpyHostToDevice, h->stream));
+  }
+  launchInterUpload(h->s, h->interAos, n_inter, h->stream);
+  HIPCHECK(h, hipGetLastError());
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   h->state_ready = true;
   h->forces_ready = false;
@@ -492,19 +583,33 @@ int misa_upload_state(misa_handle* h, const void* atoms, const void* inter, int6
 }
 
 int misa_download_state(misa_handle* h, void* atoms, void* inter, int64_t inter_cap, int64_t* n_inter) {
-  (void)inter;
-  (void)inter_cap;
   if (!h || !atoms) return MISA_E_ARG;
   if (!h->state_ready) return fail(h, MISA_E_ARG, "no state uploaded");
   HIPCHECK(h, hipSetDevice(h->device));
   const Box& b = h->box;
+  const bool mixed = h->precision == MISA_PREC_MIXED;
   const size_t bytes = static_cast<size_t>(b.slots()) * sizeof(AtomElement);
   if (!h->aos_dev) HIPCHECK(h, hipMalloc(&h->aos_dev, bytes));
-  launchDownload(h->s, h->aos_dev, b.sx, b.sy, h->precision == MISA_PREC_MIXED, h->stream);
+  launchDownload(h->s, h->aos_dev, b.sx, b.sy, mixed, h->stream);
   HIPCHECK(h, hipGetLastError());
   HIPCHECK(h, hipMemcpyAsync(atoms, h->aos_dev, bytes, hipMemcpyDeviceToHost, h->stream));
+  Ctl c;
+  HIPCHECK(h, hipMemcpyAsync(&c, h->s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
   HIPCHECK(h, hipStreamSynchronize(h->stream));
-  if (n_inter) *n_inter = 0;
+  if (n_inter) *n_inter = c.n_inter;
+  if (c.n_inter > 0) {
+    if (!inter || c.n_inter > inter_cap) return fail(h, MISA_E_CAPACITY, "inter buffer too small");
+    if (h->interAosCap < c.n_inter) {
+      if (h->interAos) HIPCHECK(h, hipFree(h->interAos));
+      h->interAos = nullptr;
+      HIPCHECK(h, hipMalloc(&h->interAos, static_cast<size_t>(h->s.in.cap) * sizeof(AtomElement)));
+      h->interAosCap = h->s.in.cap;
+    }
+    launchInterDownload(h->s, h->interAos, mixed, h->stream);
+    HIPCHECK(h, hipGetLastError());
+    HIPCHECK(h, hipMemcpyAsync(inter, h->interAos, c.n_inter * sizeof(AtomElement), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+  }
   return MISA_OK;
 }
 
@@ -542,18 +647,45 @@ int misa_set_timestep(misa_handle* h, double dt) {
 }
 
 int misa_set_inter_capacity(misa_handle* h, int64_t cap) {
-  if (!h || cap < 1) return MISA_E_ARG;
-  return MISA_OK;
+  if (!h || cap < 1 || cap >= (int64_t(1) << 21)) return MISA_E_ARG;
+  if (h->state_ready) return fail(h, MISA_E_ARG, "set the inter-atom capacity before uploading a state");
+  HIPCHECK(h, hipSetDevice(h->device));
+  return allocInter(h, cap);
+}
+
+// clearForce + computeEam (atom.cpp:86-149) on the resident state, after the
+// inter-atom migration, ghost images and index (mode 1 of the decide kernel).
+static void forcePass(misa_handle* h, int lattice_mode, int inter_mode, bool store_rho, const VerletParams& vp) {
+  const bool mixed = h->precision == MISA_PREC_MIXED;
+  {
+    Timer t(h, 2);
+    launchInterRho(h->s, h->t64, h->t32, mixed, h->stream);
+  }
+  {
+    Timer t(h, 0);
+    launchRhoDf(h->s, h->lat, h->t64, h->t32, mixed, h->acc, store_rho, h->stream);
+  }
+  {
+    Timer t(h, 2);
+    launchInterForce(h->s, h->t64, h->t32, mixed, inter_mode, vp, h->stream);
+  }
+  {
+    Timer t(h, 1);
+    launchForce(h->s, h->lat, h->t64, h->t32, mixed, h->acc, lattice_mode, vp, h->stream);
+  }
 }
 
 int misa_compute_forces(misa_handle* h) {
   if (!h) return MISA_E_ARG;
-  if (!h->state_ready || !h->have_pot || !h->have_offsets) return fail(h, MISA_E_ARG, "state, potential and offsets required");
+  if (!h->state_ready || !h->have_pot || !h->have_offsets)
+    return fail(h, MISA_E_ARG, "state, potential and offsets required");
   HIPCHECK(h, hipSetDevice(h->device));
-  const bool mixed = h->precision == MISA_PREC_MIXED;
   const VerletParams vp = verletOf(h->dt);
-  launchRhoDf(h->s, h->lat, h->t64, h->t32, mixed, h->acc, true, h->stream);
-  launchForce(h->s, h->lat, h->t64, h->t32, mixed, h->acc, 2, vp, h->stream);
+  const bool prof = h->profile;
+  h->profile = false;
+  launchDecide(h->s, 1, h->stream);
+  forcePass(h, 2, 2, true, vp);
+  h->profile = prof;
   HIPCHECK(h, hipGetLastError());
   const int rc = syncAndCheck(h);
   if (rc != MISA_OK) return rc;
@@ -566,34 +698,40 @@ int misa_step(misa_handle* h, int64_t nsteps) {
   if (!h->forces_ready) return fail(h, MISA_E_ARG, "misa_compute_forces must run before misa_step");
   if (nsteps == 0) return MISA_OK;
   HIPCHECK(h, hipSetDevice(h->device));
-  const bool mixed = h->precision == MISA_PREC_MIXED;
   const VerletParams vp = verletOf(h->dt);
   for (auto e : h->events) hipEventDestroy(e);
   h->events.clear();
+  h->event_cls.clear();
   hipEvent_t e0, e1;
   HIPCHECK(h, hipEventCreate(&e0));
   HIPCHECK(h, hipEventCreate(&e1));
   HIPCHECK(h, hipEventRecord(e0, h->stream));
-  std::vector<int> cls;
+  // firststep + decide of the first step (simulation.cpp:168-176)
   {
     Timer t(h, 2);
-    if (h->profile) cls.push_back(2);
     launchFirst(h->s, vp, h->stream);
+    launchInterFirst(h->s, vp, h->stream);
   }
   h->s.cur = 1 - h->s.cur;
+  {
+    Timer t(h, 2);
+    launchDecide(h->s, 0, h->stream);
+  }
   for (int64_t i = 0; i < nsteps; i++) {
     const bool last = i == nsteps - 1;
-    {
-      Timer t(h, 0);
-      if (h->profile) cls.push_back(0);
-      launchRhoDf(h->s, h->lat, h->t64, h->t32, mixed, h->acc, last, h->stream);
+    // forces of step i, its secondstep, and (fused) the firststep of step i+1
+    forcePass(h, last ? 0 : 1, 0, last, vp);
+    if (!last) {
+      {
+        Timer t(h, 2);
+        launchInterFirst(h->s, vp, h->stream);
+      }
+      h->s.cur = 1 - h->s.cur;
+      {
+        Timer t(h, 2);
+        launchDecide(h->s, 0, h->stream);
+      }
     }
-    {
-      Timer t(h, 1);
-      if (h->profile) cls.push_back(1);
-      launchForce(h->s, h->lat, h->t64, h->t32, mixed, h->acc, last ? 0 : 1, vp, h->stream);
-    }
-    if (!last) h->s.cur = 1 - h->s.cur;
   }
   HIPCHECK(h, hipGetLastError());
   HIPCHECK(h, hipEventRecord(e1, h->stream));
@@ -604,11 +742,11 @@ int misa_step(misa_handle* h, int64_t nsteps) {
   hipEventDestroy(e1);
   h->timings[0] = h->timings[1] = h->timings[2] = 0;
   h->launches[0] = h->launches[1] = h->launches[2] = 0;
-  for (size_t k = 0; k < cls.size(); k++) {
+  for (size_t k = 0; k < h->event_cls.size(); k++) {
     float t = 0;
     hipEventElapsedTime(&t, h->events[2 * k], h->events[2 * k + 1]);
-    h->timings[cls[k]] += t;
-    h->launches[cls[k]]++;
+    h->timings[h->event_cls[k]] += t;
+    h->launches[h->event_cls[k]]++;
   }
   h->timings[3] = ms;
   if (rc != MISA_OK) {
@@ -667,6 +805,23 @@ int misa_thermo_compute(misa_handle* h, misa_thermo* out) {
     out->force_sum[2] += p[7 * b + 5];
     vmax2 = std::max(vmax2, p[7 * b + 6]);
   }
+  {
+    const int64_t ni = launchInterThermo(h->s, h->stream);
+    HIPCHECK(h, hipGetLastError());
+    std::vector<double> q(static_cast<size_t>(5 * ni));
+    Ctl c;
+    HIPCHECK(h, hipMemcpyAsync(q.data(), h->s.partials, q.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipMemcpyAsync(&c, h->s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    for (int64_t b = 0; b < ni; b++) {
+      out->mvv += q[5 * b + 0];
+      out->force_sum[0] += q[5 * b + 1];
+      out->force_sum[1] += q[5 * b + 2];
+      out->force_sum[2] += q[5 * b + 3];
+      vmax2 = std::max(vmax2, q[5 * b + 4]);
+    }
+    out->n_inter = c.n_inter;
+  }
   out->max_speed = std::sqrt(vmax2);
   out->kinetic = out->mvv / 2;
   const uint64_t n = static_cast<uint64_t>(2 * h->box.n[0] * h->box.n[1] * h->box.n[2]);
@@ -683,6 +838,16 @@ int misa_thermo_compute(misa_handle* h, misa_thermo* out) {
       out->e_pair += q[3 * b + 1];
       out->virial += q[3 * b + 2];
     }
+    const int64_t ni = launchInterEnergy(h->s, h->t64, h->stream);
+    HIPCHECK(h, hipGetLastError());
+    std::vector<double> r(static_cast<size_t>(3 * ni));
+    HIPCHECK(h, hipMemcpyAsync(r.data(), h->s.partials, r.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    for (int64_t b = 0; b < ni; b++) {
+      out->e_embed += r[3 * b + 0];
+      out->e_pair += r[3 * b + 1];
+      out->virial += r[3 * b + 2];
+    }
     out->e_potential = out->e_embed + out->e_pair;
   }
   return MISA_OK;
@@ -694,6 +859,7 @@ int misa_rescale(misa_handle* h, double tset) {
   if (rc != MISA_OK) return rc;
   if (!(t.temperature > 0)) return fail(h, MISA_E_ARG, "zero temperature cannot be rescaled");
   launchScaleV(h->s, std::sqrt(tset / t.temperature), h->stream);
+  launchInterScaleV(h->s, std::sqrt(tset / t.temperature), h->stream);
   HIPCHECK(h, hipGetLastError());
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   return MISA_OK;

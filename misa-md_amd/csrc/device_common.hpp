@@ -53,6 +53,17 @@ struct Geom {
   Walk W;          // traversal order of the site kernels
 };
 
+// Control block in device memory (counters and error flags).
+struct Ctl {
+  int64_t n_inter;      // local inter atoms
+  int64_t n_ghost;      // ghost images
+  int64_t n_stage;      // new inter atoms staged by the Verlet kernels
+  int64_t n_prev;       // entries recorded in prev_slots
+  int64_t err_flags;    // bit 0: inter capacity, bit 1: ghost capacity, bit 2: non-finite, bit 3: species
+  int64_t n_keys;
+  int64_t pad[2];
+};
+
 // Pruned lattice-lattice neighbour list: slot deltas per centre parity, and
 // the same deltas shifted by `lo` (the most negative delta) so that a buffer
 // load can take them as its non-negative scalar byte offset from a base that

@@ -1,0 +1,139 @@
+// Device helpers for inter atoms (atoms that left their lattice site):
+// Wigner-Seitz nearest site, out-of-box flags, slot decoding and the bucket
+// index that maps a lattice slot to the inter atoms (and their periodic ghost
+// images) whose nearest site it is.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_common.hpp"
+
+namespace misa {
+
+// The reference's full neighbour lists (NeighbourIndex even/odd,
+// neighbour_index.inl:13-76) as parity-plane slot deltas; inter atoms and the
+// lattice sites around them walk all of them.
+struct FullPlaneOffsets {
+  int n[2];
+  int32_t off[2][kMaxFullOffsets];
+};
+
+// What the site kernels need to see the inter atoms.
+struct InterView {
+  const int32_t* head;      // per plane slot: first sorted entry of its bucket, or -1
+  const uint64_t* keys;     // sorted (slot << 24 | entry)
+  const int32_t* e_src;     // entry -> local list index
+  const double* e_x[3];     // entry positions (ghost images shifted)
+  const int8_t* type;       // local list species
+  const uint64_t* id;       // local list ids
+  const double* df64;       // local list F'
+  const float* df32;
+  const Ctl* ctl;           // n_inter, n_ghost
+  const FullPlaneOffsets* full;
+};
+
+constexpr int kEntryBits = 24;
+
+// ws::VORONOY (ws_utils.cpp:160-187): global doubled-x lattice coordinates of
+// the BCC site nearest to x.  Products with the +-1 normals are exact; sums
+// are kept in the reference's order.
+__device__ __forceinline__ void voronoyDev(double x0, double x1, double x2, double a, int64_t& lx, int64_t& ly,
+                                           int64_t& lz) {
+#pragma clang fp contract(off)
+  const double q0 = x0 / a, q1 = x1 / a, q2 = x2 / a;
+  lx = static_cast<int64_t>(lround(q0));
+  ly = static_cast<int64_t>(lround(q1));
+  lz = static_cast<int64_t>(lround(q2));
+  const double dx = q0 - static_cast<double>(lx);
+  const double dy = q1 - static_cast<double>(ly);
+  const double dz = q2 - static_cast<double>(lz);
+  const bool bx = dx > 0, by = dy > 0, bz = dz > 0;
+  const double s = ((bx ? dx : -dx) + (by ? dy : -dy)) + (bz ? dz : -dz);
+  lx = 2 * lx;
+  if (s + (-3.0 / 4.0) >= 0.0) {
+    lx += bx ? 1 : -1;
+    ly += by ? 0 : -1;
+    lz += bz ? 0 : -1;
+  }
+}
+
+// ws::isOutBox (ws_utils.cpp:189-213) for the single-rank box at the origin.
+__device__ __forceinline__ unsigned outBoxFlags(const Geom& G, double x0, double x1, double x2) {
+  int64_t c0, c1, c2;
+  voronoyDev(x0, x1, x2, G.a, c0, c1, c2);
+  unsigned f = 0;
+  if (c0 < 0) f |= 1u;
+  else if (c0 >= 2 * static_cast<int64_t>(G.n0)) f |= 2u;
+  if (c1 < 0) f |= 4u;
+  else if (c1 >= G.n1) f |= 8u;
+  if (c2 < 0) f |= 16u;
+  else if (c2 >= G.n2) f |= 32u;
+  return f;
+}
+
+// ghost-extended coordinates (x doubled) -> plane slot
+__device__ __forceinline__ int64_t extSlot(const Geom& G, int64_t X, int64_t y, int64_t z) {
+  return (X & 1) * G.plane + (z * G.NY + y) * G.NX + (X >> 1);
+}
+
+// ws::getNearLatCoord (ws_utils.cpp:254-261): nearest site in ghost-extended
+// coordinates; returns its slot, or -1 outside the ghost-extended box.
+__device__ __forceinline__ int64_t nearExtSlot(const Geom& G, double x0, double x1, double x2, int64_t& X, int64_t& y,
+                                               int64_t& z) {
+  voronoyDev(x0, x1, x2, G.a, X, y, z);
+  X += 2 * G.g;
+  y += G.g;
+  z += G.g;
+  if (X < 0 || y < 0 || z < 0 || X >= 2 * static_cast<int64_t>(G.NX) || y >= G.NY || z >= G.NZ) return -1;
+  return extSlot(G, X, y, z);
+}
+
+// ws::findNearLatIndexInSubBox (ws_utils.cpp:231-252): nearest site if it is
+// a sub-box site, else -1.
+__device__ __forceinline__ int64_t nearSubBoxSlot(const Geom& G, double x0, double x1, double x2) {
+  int64_t X, y, z;
+  voronoyDev(x0, x1, x2, G.a, X, y, z);
+  if (X < 0 || y < 0 || z < 0 || X >= 2 * static_cast<int64_t>(G.n0) || y >= G.n1 || z >= G.n2) return -1;
+  return extSlot(G, X + 2 * G.g, y + G.g, z + G.g);
+}
+
+__device__ __forceinline__ void decodeSlot(const Geom& G, int64_t slot, int& p, int& cx, int& cy, int& cz) {
+  p = slot >= G.plane ? 1 : 0;
+  int64_t r = slot - p * G.plane;
+  cx = static_cast<int>(r % G.NX) - G.g;
+  r /= G.NX;
+  cy = static_cast<int>(r % G.NY) - G.g;
+  cz = static_cast<int>(r / G.NY) - G.g;
+}
+
+__device__ __forceinline__ bool isSubBoxSlot(const Geom& G, int64_t slot) {
+  int p, cx, cy, cz;
+  decodeSlot(G, slot, p, cx, cy, cz);
+  return cx >= 0 && cx < G.n0 && cy >= 0 && cy < G.n1 && cz >= 0 && cz < G.n2;
+}
+
+template <typename T>
+__device__ __forceinline__ T dfOf(const InterView& iv, int src);
+template <>
+__device__ __forceinline__ double dfOf<double>(const InterView& iv, int src) {
+  return iv.df64[src];
+}
+template <>
+__device__ __forceinline__ float dfOf<float>(const InterView& iv, int src) {
+  return iv.df32[src];
+}
+
+// Visits the entries of the bucket of `slot` in insertion order.
+template <typename Fn>
+__device__ __forceinline__ void forBucket(const InterView& iv, int64_t slot, int ne, Fn fn) {
+  const int h = iv.head[slot];
+  if (h < 0) return;
+  for (int q = h; q < ne; ++q) {
+    const uint64_t k = iv.keys[q];
+    if (static_cast<int64_t>(k >> kEntryBits) != slot) break;
+    fn(static_cast<int>(k & ((1u << kEntryBits) - 1)));
+  }
+}
+
+}  // namespace misa

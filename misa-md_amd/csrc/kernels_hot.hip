@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
+#include "inter_common.hpp"
 #include "resident.hpp"
 #include "site_common.hpp"
 
@@ -110,7 +111,7 @@ __device__ __forceinline__ float rsqT(float x) { return __builtin_amdgcn_rsqf(x)
 template <typename T, int ACC, int U>
 __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T> tb, CXPtr x,
                                                  const int8_t* __restrict__ type, T* __restrict__ df,
-                                                 double* __restrict__ rho_out) {
+                                                 double* __restrict__ rho_out, InterView iv) {
   int64_t s;
   int p, cx, cy, cz;
   if (!siteOfThread(G, s, p, cx, cy, cz)) return;
@@ -161,6 +162,19 @@ __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T>
     int k = 0;
     for (; k + U <= nk; k += U) group(std::integral_constant<int, U>{}, k);
     for (; k < nk; ++k) group(std::integral_constant<int, 1>{}, k);
+    if (ti & kNearInter) {  // inter atoms whose nearest site is within the full list (atom.cpp:224-240)
+      const int ne = static_cast<int>(iv.ctl->n_inter + iv.ctl->n_ghost);
+      const FullPlaneOffsets& full = *iv.full;
+      auto bucket = [&](int64_t q) {
+        forBucket(iv, q, ne, [&](int e) {
+          const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
+          const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+          if (r2 < G.rc2) acc.add(rhoPair(tb, species(iv.type[iv.e_src[e]]), r2));
+        });
+      };
+      bucket(s);
+      for (int q = 0; q < full.n[p]; q++) bucket(s + full.off[p][q]);
+    }
     rho = acc.value();
     dfi = dEmbed(tb, species(ti), static_cast<T>(rho));
   }
@@ -186,14 +200,18 @@ template <typename T, int ACC, int MODE, int U>
 __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T> tb, CXPtr x, XPtr xn,
                                                  const int8_t* __restrict__ type, const T* __restrict__ df,
                                                  XPtr v, XPtr f, const uint64_t* __restrict__ id,
-                                                 VerletParams vp, StageArgs st, Ctl* ctl) {
+                                                 VerletParams vp, StageArgs st, Ctl* ctl, InterView iv) {
   int64_t s;
   int p, cx, cy, cz;
   if (!siteOfThread(G, s, p, cx, cy, cz)) return;
   const int8_t ti = type[s];
   const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
   if (isVacant(ti)) {  // vacancy: keeps its stale position (atom.cpp:43-47)
-    if (MODE == 1) {
+    if (MODE != 1) {   // clearForce left 0 there and latForce skips it (atom.cpp:86-100, 311-358)
+      f.c[0][s] = 0.0;
+      f.c[1][s] = 0.0;
+      f.c[2][s] = 0.0;
+    } else {
       xn.c[0][s] = xi0;
       xn.c[1][s] = xi1;
       xn.c[2][s] = xi2;
@@ -282,6 +300,20 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
   int k = 0;
   for (; k + U <= nk; k += U) group(std::integral_constant<int, U>{}, k);
   for (; k < nk; ++k) group(std::integral_constant<int, 1>{}, k);
+  if (ti & kNearInter) {  // forces from inter atoms on lattice neighbours (atom.cpp:397-422)
+    const int ne = static_cast<int>(iv.ctl->n_inter + iv.ctl->n_ghost);
+    const FullPlaneOffsets& full = *iv.full;
+    auto bucket = [&](int64_t q) {
+      forBucket(iv, q, ne, [&](int e) {
+        const int src = iv.e_src[e];
+        const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        if (r2 < G.rc2) acc.add(d0, d1, d2, pairForce(tb, ei, species(iv.type[src]), r2, dfi, dfOf<T>(iv, src)));
+      });
+    };
+    bucket(s);
+    for (int q = 0; q < full.n[p]; q++) bucket(s + full.off[p][q]);
+  }
   double f0, f1, f2;
   sum3Value<ACC>(acc, f0, f1, f2);
   const double c = vp.c[ei];
@@ -335,12 +367,13 @@ void launchRhoDf(const DevState& s, const LatOffsets& o, const Tables<double>& t
                  bool mixed, int acc, bool store_rho, hipStream_t st) {
   double* rho = store_rho ? s.rho : nullptr;
   const unsigned grid = siteGrid(s.G);
+  const InterView iv = interView(s, mixed);
   if (!mixed)
-    kRhoDf<double, 0, 4><<<grid, kBlock, 0, st>>>(s.G, o, t64, cxp(s.x[s.cur]), s.type, s.df64, rho);
+    kRhoDf<double, 0, 4><<<grid, kBlock, 0, st>>>(s.G, o, t64, cxp(s.x[s.cur]), s.type, s.df64, rho, iv);
   else if (acc == 0)
-    kRhoDf<float, 0, kUFloat><<<grid, kBlock, 0, st>>>(s.G, o, t32, cxp(s.x[s.cur]), s.type, s.df32, rho);
+    kRhoDf<float, 0, kUFloat><<<grid, kBlock, 0, st>>>(s.G, o, t32, cxp(s.x[s.cur]), s.type, s.df32, rho, iv);
   else
-    kRhoDf<float, 1, kUFloat><<<grid, kBlock, 0, st>>>(s.G, o, t32, cxp(s.x[s.cur]), s.type, s.df32, rho);
+    kRhoDf<float, 1, kUFloat><<<grid, kBlock, 0, st>>>(s.G, o, t32, cxp(s.x[s.cur]), s.type, s.df32, rho, iv);
 }
 
 void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t64, const Tables<float>& t32,
@@ -349,8 +382,9 @@ void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t
   const CXPtr x = cxp(s.x[s.cur]);
   const XPtr xn = xp(s.x[1 - s.cur]);
   const StageArgs sa = stageOf(s);
+  const InterView iv = interView(s, mixed);
 #define MISA_FORCE(T, A, M, U, TB, DF) \
-  kForce<T, A, M, U><<<grid, kBlock, 0, st>>>(s.G, o, TB, x, xn, s.type, DF, xp(s.v), xp(s.f), s.id, vp, sa, s.ctl)
+  kForce<T, A, M, U><<<grid, kBlock, 0, st>>>(s.G, o, TB, x, xn, s.type, DF, xp(s.v), xp(s.f), s.id, vp, sa, s.ctl, iv)
   if (!mixed) {
     if (mode == 0) MISA_FORCE(double, 0, 0, kUDouble, t64, s.df64);
     else if (mode == 1) MISA_FORCE(double, 0, 1, kUDouble, t64, s.df64);

@@ -1,0 +1,725 @@
+// Inter atoms on the device: ownership decisions, migration, periodic ghost
+// images, the nearest-site bucket index, and their density, F', force and
+// velocity-Verlet steps.
+//
+// Reference behaviour restated (single rank, periodic box):
+//   atom::decide            atom.cpp:21-84
+//   InterAtomList           inter_atom_list.cpp:16-75 (list order, makeIndex)
+//   exchangeInter           inter_particle_packer.cpp:17-121
+//   borderInter             inter_border_packer.cpp:12-106
+//   atom::interRho          atom.cpp:194-284
+//   atom::interForce        atom.cpp:360-473
+//   NewtonMotion (inter)    newton_motion.cpp:46-54, 68-73
+//
+// The list is kept in the reference's order: new inter atoms are appended in
+// the z->y->x order of the sites they left, recombined atoms are removed in
+// place, migrating atoms are moved to the end (dimension by dimension, the
+// atoms sent to the lower neighbour before those sent to the upper one), and
+// ghost images follow the local atoms in the order borderInter appends them.
+// Bucket order is insertion order, so every per-atom sum is taken in the
+// reference's order.  Recombination claims a vacancy for the lowest list
+// index among the atoms that target it, which is what the reference's
+// sequential loop yields, since vacancies are only created before that loop.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "device_common.hpp"
+#include "inter_common.hpp"
+#include "resident.hpp"
+#include "site_common.hpp"
+
+namespace misa {
+
+namespace {
+
+using namespace site;
+
+constexpr int kDT = 1024;  // threads of the single-workgroup decide kernel
+constexpr int kIT = 256;   // threads per block of the per-atom kernels
+constexpr int kIB = 64;    // blocks of the per-atom kernels (grid-stride)
+
+struct DecideArgs {
+  Geom G;
+  int mode;
+  int cur;
+  double* lx[3];  // lattice positions (current buffer)
+  double* lv[3];
+  int8_t* ltype;
+  uint64_t* lid;
+  InterDev in;
+  Ctl* ctl;
+};
+
+// Exclusive block-wide prefix sum of v (1024 threads); `total` gets the sum.
+__device__ int blockScan(int v, int& total) {
+  __shared__ int wsum[kDT / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    int t = lane < kDT / 64 ? wsum[lane] : 0;
+#pragma unroll
+    for (int o = 1; o < kDT / 64; o <<= 1) {
+      const int y = __shfl_up(t, o, 64);
+      if (lane >= o) t += y;
+    }
+    if (lane < kDT / 64) wsum[lane] = t;
+  }
+  __syncthreads();
+  const int before = w > 0 ? wsum[w - 1] : 0;
+  total = wsum[kDT / 64 - 1];
+  __syncthreads();
+  return before + x - v;
+}
+
+__device__ __forceinline__ void setFlag(unsigned long long* f, unsigned long long bit) { atomicOr(f, bit); }
+
+// Copy local atom src (main arrays) to scratch slot dst, shifting x[d] by sh.
+__device__ __forceinline__ void toScratch(InterDev& in, int src, int dst, int d, double sh) {
+  in.id2[dst] = in.id[src];
+  in.type2[dst] = in.type[src];
+  for (int k = 0; k < 3; k++) {
+    in.x2[k][dst] = in.x[k][src] + (k == d ? sh : 0.0);
+    in.v2[k][dst] = in.v[k][src];
+  }
+}
+
+__device__ __forceinline__ void fromScratch(InterDev& in, int n) {
+  for (int i = threadIdx.x; i < n; i += kDT) {
+    in.id[i] = in.id2[i];
+    in.type[i] = in.type2[i];
+    for (int k = 0; k < 3; k++) {
+      in.x[k][i] = in.x2[k][i];
+      in.v[k][i] = in.v2[k][i];
+    }
+  }
+  __syncthreads();
+}
+
+// Forwarding region of borderInter in ghost-extended coordinates (x doubled):
+// the ghost-wide layer of the sub-box next to face (dim, dir); dimensions
+// already exchanged span the whole ghost-extended range.
+__device__ __forceinline__ bool inForwardRegion(const Geom& G, int dim, int dir, int64_t X, int64_t y, int64_t z) {
+  const int64_t c[3] = {X, y, z};
+  const int64_t g[3] = {2 * G.g, G.g, G.g};
+  const int64_t s[3] = {2 * static_cast<int64_t>(G.n0), G.n1, G.n2};
+  const int64_t e[3] = {2 * static_cast<int64_t>(G.NX), G.NY, G.NZ};
+  for (int k = 0; k < 3; k++) {
+    int64_t lo, hi;
+    if (k == dim) {
+      lo = dir == 0 ? g[k] : s[k];
+      hi = dir == 0 ? 2 * g[k] : s[k] + g[k];
+    } else if (k < dim) {
+      lo = 0;
+      hi = e[k];
+    } else {
+      lo = g[k];
+      hi = g[k] + s[k];
+    }
+    if (c[k] < lo || c[k] >= hi) return false;
+  }
+  return true;
+}
+
+__device__ __forceinline__ void writeLatticeImages(const DecideArgs& A, int64_t slot, bool pos, int8_t t) {
+  int p, cx, cy, cz;
+  decodeSlot(A.G, slot, p, cx, cy, cz);
+  const double y0 = A.lx[0][slot], y1 = A.lx[1][slot], y2 = A.lx[2][slot];
+  forEachImage(A.G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
+    A.ltype[q] = t;
+    if (pos) {
+      A.lx[0][q] = y0 + sx * A.G.L0;
+      A.lx[1][q] = y1 + sy * A.G.L1;
+      A.lx[2][q] = y2 + sz * A.G.L2;
+    }
+  });
+}
+
+__global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
+  const Geom& G = A.G;
+  InterDev in = A.in;
+  const int tid = threadIdx.x;
+  int nl = static_cast<int>(A.ctl->n_inter);
+  const int nprev = static_cast<int>(A.ctl->n_prev);
+  __syncthreads();
+  unsigned long long* err = reinterpret_cast<unsigned long long*>(&A.ctl->err_flags);
+
+  if (A.mode == 0) {
+    // P1: append the new inter atoms in the order of the sites they left
+    // (atom.cpp:28-53 walks the sub-box z -> y -> x and appends)
+    int ns = static_cast<int>(A.ctl->n_stage);
+    if (ns > in.cap) ns = static_cast<int>(in.cap);
+    if (nl + ns > in.cap) {
+      if (tid == 0) setFlag(err, 1ull);
+      ns = static_cast<int>(in.cap) - nl;
+    }
+    for (int i = tid; i < ns; i += kDT) {
+      const int64_t key = in.stage_key[i];
+      int rank = 0;
+      for (int j = 0; j < ns; j++) rank += in.stage_key[j] < key ? 1 : 0;
+      const int dst = nl + rank;
+      in.id[dst] = in.stage_id[i];
+      in.type[dst] = static_cast<int8_t>(species(in.stage_type[i]));
+      for (int k = 0; k < 3; k++) {
+        in.x[k][dst] = in.stage_x[k][i];
+        in.v[k][dst] = in.stage_v[k][i];
+        in.f[k][dst] = 0.0;
+      }
+      const int64_t slot = in.stage_slot[i];
+      A.ltype[slot] = kVacant;  // type INVALID; its stale x stays (atom.cpp:43-47)
+      writeLatticeImages(A, slot, false, kVacant);
+    }
+    __syncthreads();
+    nl += ns;
+
+    // P2: an inter atom whose Wigner-Seitz site is a vacant sub-box site whose
+    // stale position is in the box refills it (atom.cpp:55-83)
+    for (int i = tid; i < nl; i += kDT) {
+      const int64_t t = nearSubBoxSlot(G, in.x[0][i], in.x[1][i], in.x[2][i]);
+      int target = -1;
+      if (t >= 0 && isVacant(A.ltype[t]) && outBoxFlags(G, A.lx[0][t], A.lx[1][t], A.lx[2][t]) == 0u) {
+        atomicMin(&in.site_claim[t], i);
+        target = static_cast<int>(t);
+      }
+      in.tmp[i] = target;
+    }
+    __syncthreads();
+    for (int i = tid; i < nl; i += kDT) {
+      const int t = in.tmp[i];
+      if (t >= 0 && in.site_claim[t] == i) {
+        A.lid[t] = in.id[i];
+        A.ltype[t] = in.type[i];
+        for (int k = 0; k < 3; k++) {
+          A.lx[k][t] = in.x[k][i];
+          A.lv[k][t] = in.v[k][i];
+        }
+        writeLatticeImages(A, t, true, in.type[i]);
+        in.tmp[i] = -2;  // removed
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < nl; i += kDT) {
+      const int t = in.tmp[i];
+      if (t >= 0) in.site_claim[t] = INT_MAX;
+    }
+    __syncthreads();
+
+    // P3: remove the recombined atoms, keeping the list order
+    int kept = 0;
+    for (int base = 0; base < nl; base += kDT) {
+      const int i = base + tid;
+      const int keep = (i < nl && in.tmp[i] != -2) ? 1 : 0;
+      int tot;
+      const int ex = blockScan(keep, tot);
+      if (keep) toScratch(in, i, kept + ex, -1, 0.0);
+      kept += tot;
+    }
+    __syncthreads();
+    nl = kept;
+    fromScratch(in, nl);
+  }
+
+  // P4: exchangeInter: per dimension, atoms out of the box below / above
+  // move to the end of the list, shifted by +L / -L (inter_particle_packer.cpp)
+  const double L[3] = {G.L0, G.L1, G.L2};
+  for (int d = 0; d < 3; d++) {
+    const unsigned lowBit = 1u << (2 * d), highBit = 2u << (2 * d);
+    for (int i = tid; i < nl; i += kDT) {
+      const unsigned fl = outBoxFlags(G, in.x[0][i], in.x[1][i], in.x[2][i]);
+      in.tmp[i] = (fl & lowBit) ? 1 : ((fl & highBit) ? 2 : 0);
+    }
+    __syncthreads();
+    int cnt[3] = {0, 0, 0};
+    for (int grp = 0; grp < 3; grp++) {
+      int run = 0;
+      for (int base = 0; base < nl; base += kDT) {
+        const int i = base + tid;
+        const int is = (i < nl && in.tmp[i] == grp) ? 1 : 0;
+        int tot;
+        const int ex = blockScan(is, tot);
+        if (is) {
+          const int dst = (grp == 0 ? 0 : (grp == 1 ? cnt[0] : cnt[0] + cnt[1])) + run + ex;
+          toScratch(in, i, dst, grp == 0 ? -1 : d, grp == 1 ? L[d] : (grp == 2 ? -L[d] : 0.0));
+        }
+        run += tot;
+      }
+      cnt[grp] = run;
+    }
+    __syncthreads();
+    fromScratch(in, nl);
+  }
+
+  // P5: entries = local atoms, then borderInter's ghost images
+  for (int i = tid; i < nl; i += kDT) {
+    in.e_src[i] = i;
+    for (int k = 0; k < 3; k++) in.e_x[k][i] = in.x[k][i];
+  }
+  __syncthreads();
+  int ne = nl;
+  for (int d = 0; d < 3; d++) {
+    const int ne0 = ne;
+    for (int dir = 0; dir < 2; dir++) {
+      const double sh = dir == 0 ? L[d] : -L[d];
+      for (int base = 0; base < ne0; base += kDT) {
+        const int e = base + tid;
+        int is = 0;
+        if (e < ne0) {
+          int64_t X, y, z;
+          voronoyDev(in.e_x[0][e], in.e_x[1][e], in.e_x[2][e], G.a, X, y, z);
+          is = inForwardRegion(G, d, dir, X + 2 * G.g, y + G.g, z + G.g) ? 1 : 0;
+        }
+        int tot;
+        const int ex = blockScan(is, tot);
+        if (is) {
+          const int dst = ne + ex;
+          if (dst < in.ecap) {
+            in.e_src[dst] = in.e_src[e];
+            for (int k = 0; k < 3; k++) in.e_x[k][dst] = in.e_x[k][e] + (k == d ? sh : 0.0);
+          } else {
+            setFlag(err, 2ull);
+          }
+        }
+        ne += tot;
+      }
+      if (ne > in.ecap) ne = static_cast<int>(in.ecap);
+      __syncthreads();
+    }
+  }
+
+  // P6: reset the previous index and the flags around its entries
+  for (int q = tid; q < nprev; q += kDT) {
+    const int64_t slot = in.prev_slot[q];
+    in.head[slot] = -1;
+  }
+  __syncthreads();
+  const int nfull = 1 + max(in.full->n[0], in.full->n[1]);
+  for (int q = tid; q < nprev * nfull; q += kDT) {
+    const int e = q / nfull;
+    const int k = q % nfull - 1;
+    const int64_t slot = in.prev_slot[e];
+    const int par = slot >= G.plane ? 1 : 0;
+    if (k >= in.full->n[par]) continue;
+    const int64_t t = k < 0 ? slot : slot + in.full->off[par][k];
+    if (t >= 0 && t < 2 * G.plane && isSubBoxSlot(G, t)) {
+      const int8_t v = A.ltype[t];
+      if (v & kNearInter) A.ltype[t] = static_cast<int8_t>(v & ~kNearInter);
+    }
+  }
+  __syncthreads();
+  // keys = (slot << 24 | entry), sorted: buckets in slot order, insertion order within
+  int P = 1;
+  while (P < ne) P <<= 1;
+  for (int e = tid; e < P; e += kDT) {
+    uint64_t key = ~0ull;
+    if (e < ne) {
+      int64_t X, y, z;
+      int64_t slot = nearExtSlot(G, in.e_x[0][e], in.e_x[1][e], in.e_x[2][e], X, y, z);
+      if (slot < 0) {
+        setFlag(err, 8ull);
+        slot = 0;
+      }
+      in.e_slot[e] = slot;
+      key = (static_cast<uint64_t>(slot) << kEntryBits) | static_cast<uint64_t>(e);
+    }
+    in.keys[e] = key;
+  }
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = tid; i < P; i += kDT) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const uint64_t a = in.keys[i], b = in.keys[ixj];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            in.keys[i] = b;
+            in.keys[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int q = tid; q < ne; q += kDT) {
+    const int64_t slot = static_cast<int64_t>(in.keys[q] >> kEntryBits);
+    if (q == 0 || static_cast<int64_t>(in.keys[q - 1] >> kEntryBits) != slot) in.head[slot] = q;
+    in.prev_slot[q] = slot;
+  }
+  __syncthreads();
+  // P7: flag the sub-box sites within the full neighbour list of every entry
+  for (int q = tid; q < ne * nfull; q += kDT) {
+    const int e = q / nfull;
+    const int k = q % nfull - 1;
+    const int64_t slot = in.e_slot[e];
+    const int par = slot >= G.plane ? 1 : 0;
+    if (k >= in.full->n[par]) continue;
+    const int64_t t = k < 0 ? slot : slot + in.full->off[par][k];
+    if (t >= 0 && t < 2 * G.plane && isSubBoxSlot(G, t)) {
+      const int8_t v = A.ltype[t];
+      if (!(v & kNearInter)) A.ltype[t] = static_cast<int8_t>(v | kNearInter);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    A.ctl->n_inter = nl;
+    A.ctl->n_ghost = ne - nl;
+    A.ctl->n_stage = 0;
+    A.ctl->n_prev = ne;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per inter atom: rho/F', force + half kick, first half step
+// ---------------------------------------------------------------------------
+
+// atom::interRho (atom.cpp:194-284): lattice (nearest site, full list), same
+// bucket without itself, then the buckets of the full list.
+template <typename T>
+__global__ __launch_bounds__(kIT) void kInterRho(Geom G, Tables<T> tb, InterView iv, CXPtr lx,
+                                                 const int8_t* __restrict__ ltype, InterDev in) {
+  const int nl = static_cast<int>(iv.ctl->n_inter);
+  const int ne = nl + static_cast<int>(iv.ctl->n_ghost);
+  const FullPlaneOffsets& full = *iv.full;
+  for (int i = blockIdx.x * kIT + threadIdx.x; i < nl; i += kIT * kIB) {
+    const int64_t n = in.e_slot[i];
+    if (!isSubBoxSlot(G, n)) continue;
+    const double xi0 = in.x[0][i], xi1 = in.x[1][i], xi2 = in.x[2][i];
+    const uint64_t myid = in.id[i];
+    double rho = 0.0;
+    const int par = n >= G.plane ? 1 : 0;
+    auto lat = [&](int64_t j) {
+      const int8_t tj = ltype[j];
+      if (isVacant(tj)) return;
+      const double d0 = xi0 - lx.c[0][j], d1 = xi1 - lx.c[1][j], d2 = xi2 - lx.c[2][j];
+      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      if (r2 < G.rc2) rho += rhoPair(tb, species(tj), r2);
+    };
+    auto ent = [&](int e, bool skip_self) {
+      const int src = iv.e_src[e];
+      const uint64_t eid = e < nl ? iv.id[src] : 0;  // ghost images carry id 0
+      if (skip_self && eid == myid) return;
+      const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
+      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      if (r2 < G.rc2) rho += rhoPair(tb, species(iv.type[src]), r2);
+    };
+    lat(n);
+    for (int k = 0; k < full.n[par]; k++) lat(n + full.off[par][k]);
+    forBucket(iv, n, ne, [&](int e) { ent(e, true); });
+    for (int k = 0; k < full.n[par]; k++) forBucket(iv, n + full.off[par][k], ne, [&](int e) { ent(e, false); });
+    in.rho[i] = rho;
+    const T dfi = dEmbed(tb, species(in.type[i]), static_cast<T>(rho));
+    if constexpr (sizeof(T) == 8)
+      in.df64[i] = dfi;
+    else
+      in.df32[i] = dfi;
+  }
+}
+
+// atom::interForce (atom.cpp:360-473) + secondstep for the inter list.
+template <typename T, int MODE>
+__global__ __launch_bounds__(kIT) void kInterForce(Geom G, Tables<T> tb, InterView iv, CXPtr lx,
+                                                   const int8_t* __restrict__ ltype, const T* __restrict__ ldf,
+                                                   InterDev in, VerletParams vp) {
+  const int nl = static_cast<int>(iv.ctl->n_inter);
+  const int ne = nl + static_cast<int>(iv.ctl->n_ghost);
+  const FullPlaneOffsets& full = *iv.full;
+  for (int i = blockIdx.x * kIT + threadIdx.x; i < nl; i += kIT * kIB) {
+    const int64_t n = in.e_slot[i];
+    double f0 = 0, f1 = 0, f2 = 0;
+    const int ei = species(in.type[i]);
+    if (isSubBoxSlot(G, n)) {
+      const double xi0 = in.x[0][i], xi1 = in.x[1][i], xi2 = in.x[2][i];
+      const uint64_t myid = in.id[i];
+      const T dfi = dfOf<T>(iv, i);
+      const int par = n >= G.plane ? 1 : 0;
+      auto lat = [&](int64_t j) {
+        const int8_t tj = ltype[j];
+        const double d0 = xi0 - lx.c[0][j], d1 = xi1 - lx.c[1][j], d2 = xi2 - lx.c[2][j];
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        if (r2 < G.rc2 && !isVacant(tj)) {
+          const double fp = static_cast<double>(pairForce(tb, ei, species(tj), r2, dfi, ldf[j]));
+          f0 += d0 * fp;
+          f1 += d1 * fp;
+          f2 += d2 * fp;
+        }
+      };
+      auto ent = [&](int e, bool skip_self) {
+        const int src = iv.e_src[e];
+        const uint64_t eid = e < nl ? iv.id[src] : 0;
+        if (skip_self && eid == myid) return;
+        const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        if (r2 < G.rc2) {
+          const double fp = static_cast<double>(pairForce(tb, ei, species(iv.type[src]), r2, dfi, dfOf<T>(iv, src)));
+          f0 += d0 * fp;
+          f1 += d1 * fp;
+          f2 += d2 * fp;
+        }
+      };
+      lat(n);
+      for (int k = 0; k < full.n[par]; k++) lat(n + full.off[par][k]);
+      forBucket(iv, n, ne, [&](int e) { ent(e, true); });
+      for (int k = 0; k < full.n[par]; k++) forBucket(iv, n + full.off[par][k], ne, [&](int e) { ent(e, false); });
+    }
+    in.f[0][i] = f0;
+    in.f[1][i] = f1;
+    in.f[2][i] = f2;
+    if (MODE == 0) {
+      const double c = vp.c[ei];
+      in.v[0][i] += c * f0;
+      in.v[1][i] += c * f1;
+      in.v[2][i] += c * f2;
+    }
+  }
+}
+
+// NewtonMotion::firststep for the inter list (newton_motion.cpp:46-54).
+__global__ __launch_bounds__(kIT) void kInterFirst(InterDev in, const Ctl* ctl, VerletParams vp) {
+  const int nl = static_cast<int>(ctl->n_inter);
+  for (int i = blockIdx.x * kIT + threadIdx.x; i < nl; i += kIT * kIB) {
+    const double c = vp.c[species(in.type[i])];
+    for (int d = 0; d < 3; d++) {
+      const double v = in.v[d][i] + c * in.f[d][i];
+      in.v[d][i] = v;
+      in.x[d][i] += vp.dt * v;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kIT) void kInterUpload(InterDev in, const AtomElement* __restrict__ a, int64_t n,
+                                                    Ctl* ctl) {
+  for (int64_t i = blockIdx.x * kIT + threadIdx.x; i < n; i += kIT * kIB) {
+    in.id[i] = a[i].id;
+    in.type[i] = static_cast<int8_t>(a[i].type);
+    for (int k = 0; k < 3; k++) {
+      in.x[k][i] = a[i].x[k];
+      in.v[k][i] = a[i].v[k];
+      in.f[k][i] = a[i].f[k];
+    }
+    in.rho[i] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->n_inter = n;
+    ctl->n_ghost = 0;
+    ctl->n_stage = 0;
+  }
+}
+
+__global__ __launch_bounds__(kIT) void kInterDownload(InterDev in, AtomElement* __restrict__ a, const Ctl* ctl,
+                                                      bool mixed) {
+  const int64_t n = ctl->n_inter;
+  for (int64_t i = blockIdx.x * kIT + threadIdx.x; i < n; i += kIT * kIB) {
+    AtomElement e;
+    e.id = in.id[i];
+    e.type = in.type[i];
+    for (int k = 0; k < 3; k++) {
+      e.x[k] = in.x[k][i];
+      e.v[k] = in.v[k][i];
+      e.f[k] = in.f[k][i];
+    }
+    e.rho = in.rho[i];
+    e.df = mixed ? static_cast<double>(in.df32[i]) : in.df64[i];
+    a[i] = e;
+  }
+}
+
+__global__ __launch_bounds__(kIT) void kInterScaleV(InterDev in, const Ctl* ctl, double factor) {
+  const int nl = static_cast<int>(ctl->n_inter);
+  for (int i = blockIdx.x * kIT + threadIdx.x; i < nl; i += kIT * kIB)
+    for (int d = 0; d < 3; d++) in.v[d][i] *= factor;
+}
+
+template <int N>
+__device__ __forceinline__ void blockSumI(double (&vals)[N], double* out) {
+  __shared__ double red[N][kIT / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < N; q++) {
+    double v = vals[q];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) red[q][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+      double v = 0;
+      for (int k = 0; k < kIT / 64; k++) v += red[q][k];
+      out[q] = v;
+    }
+}
+
+__global__ __launch_bounds__(kIT) void kInterThermo(InterDev in, const Ctl* ctl, double* partials) {
+  const int nl = static_cast<int>(ctl->n_inter);
+  double vals[5] = {0, 0, 0, 0, 0};
+  double vmax = 0;
+  for (int i = blockIdx.x * kIT + threadIdx.x; i < nl; i += kIT * kIB) {
+    const int t = species(in.type[i]);
+    const double m = t == 0 ? 55.845 : (t == 1 ? 63.546 : 58.6934);
+    const double a = in.v[0][i], b = in.v[1][i], c = in.v[2][i];
+    const double vv = a * a + b * b + c * c;
+    vals[0] += vv * m;
+    vals[1] += in.f[0][i];
+    vals[2] += in.f[1][i];
+    vals[3] += in.f[2][i];
+    vmax = fmax(vmax, vv);
+  }
+  __shared__ double mx[kIT];
+  mx[threadIdx.x] = vmax;
+  blockSumI<5>(vals, partials + 5 * blockIdx.x);
+  __syncthreads();
+  for (int o = kIT / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[5 * blockIdx.x + 4] = mx[0];
+}
+
+// Energy of the inter atoms: F(rho_I) + phi to lattice neighbours (each such
+// pair once, from the inter side) + 1/2 phi to inter neighbours; virial alike.
+__global__ __launch_bounds__(kIT) void kInterEnergy(Geom G, Tables<double> tb, InterView iv, CXPtr lx,
+                                                    const int8_t* __restrict__ ltype, const double* __restrict__ lrho,
+                                                    InterDev in, double* partials) {
+  const int nl = static_cast<int>(iv.ctl->n_inter);
+  const int ne = nl + static_cast<int>(iv.ctl->n_ghost);
+  const FullPlaneOffsets& full = *iv.full;
+  double vals[3] = {0, 0, 0};
+  for (int i = blockIdx.x * kIT + threadIdx.x; i < nl; i += kIT * kIB) {
+    const int64_t n = in.e_slot[i];
+    if (!isSubBoxSlot(G, n)) continue;
+    const int ei = species(in.type[i]);
+    const double ri = in.rho[i];
+    const double dfi = dEmbed(tb, ei, ri);
+    vals[0] += embed(tb, ei, ri);
+    const double xi0 = in.x[0][i], xi1 = in.x[1][i], xi2 = in.x[2][i];
+    const uint64_t myid = in.id[i];
+    const int par = n >= G.plane ? 1 : 0;
+    auto lat = [&](int64_t j) {
+      const int8_t tj = ltype[j];
+      const double d0 = xi0 - lx.c[0][j], d1 = xi1 - lx.c[1][j], d2 = xi2 - lx.c[2][j];
+      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      if (r2 < G.rc2 && !isVacant(tj)) {
+        const int ej = species(tj);
+        const double r = sqrt(r2);
+        double q;
+        const int m = knotOf(r, tb.rdr, tb.nr, false, q);
+        vals[1] += splineVal(tb.z2 + pairIdx(ei, ej) * tb.sR + static_cast<int64_t>(m) * kKnot, q) / r;
+        vals[2] += r2 * pairForce(tb, ei, ej, r2, dfi, dEmbed(tb, ej, lrho[j]));
+      }
+    };
+    auto ent = [&](int e, bool skip_self) {
+      const int src = iv.e_src[e];
+      const uint64_t eid = e < nl ? iv.id[src] : 0;
+      if (skip_self && eid == myid) return;
+      const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
+      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      if (r2 < G.rc2) {
+        const int ej = species(iv.type[src]);
+        const double r = sqrt(r2);
+        double q;
+        const int m = knotOf(r, tb.rdr, tb.nr, false, q);
+        vals[1] += 0.5 * splineVal(tb.z2 + pairIdx(ei, ej) * tb.sR + static_cast<int64_t>(m) * kKnot, q) / r;
+        vals[2] += 0.5 * r2 * pairForce(tb, ei, ej, r2, dfi, dEmbed(tb, ej, in.rho[src]));
+      }
+    };
+    lat(n);
+    for (int k = 0; k < full.n[par]; k++) lat(n + full.off[par][k]);
+    forBucket(iv, n, ne, [&](int e) { ent(e, true); });
+    for (int k = 0; k < full.n[par]; k++) forBucket(iv, n + full.off[par][k], ne, [&](int e) { ent(e, false); });
+  }
+  blockSumI<3>(vals, partials + 3 * blockIdx.x);
+}
+
+}  // namespace
+
+InterView interView(const DevState& s, bool mixed) {
+  InterView v;
+  v.head = s.in.head;
+  v.keys = s.in.keys;
+  v.e_src = s.in.e_src;
+  for (int k = 0; k < 3; k++) v.e_x[k] = s.in.e_x[k];
+  v.type = s.in.type;
+  v.id = s.in.id;
+  v.df64 = s.in.df64;
+  v.df32 = s.in.df32;
+  v.ctl = s.ctl;
+  v.full = s.in.full;
+  (void)mixed;
+  return v;
+}
+
+void launchDecide(const DevState& s, int mode, hipStream_t st) {
+  DecideArgs A;
+  A.G = s.G;
+  A.mode = mode;
+  A.cur = s.cur;
+  for (int k = 0; k < 3; k++) {
+    A.lx[k] = s.x[s.cur][k];
+    A.lv[k] = s.v[k];
+  }
+  A.ltype = s.type;
+  A.lid = s.id;
+  A.in = s.in;
+  A.ctl = s.ctl;
+  kDecide<<<1, kDT, 0, st>>>(A);
+}
+
+void launchInterRho(const DevState& s, const Tables<double>& t64, const Tables<float>& t32, bool mixed,
+                    hipStream_t st) {
+  const InterView iv = interView(s, mixed);
+  if (mixed)
+    kInterRho<float><<<kIB, kIT, 0, st>>>(s.G, t32, iv, cxp(s.x[s.cur]), s.type, s.in);
+  else
+    kInterRho<double><<<kIB, kIT, 0, st>>>(s.G, t64, iv, cxp(s.x[s.cur]), s.type, s.in);
+}
+
+void launchInterForce(const DevState& s, const Tables<double>& t64, const Tables<float>& t32, bool mixed, int mode,
+                      const VerletParams& vp, hipStream_t st) {
+  const InterView iv = interView(s, mixed);
+  if (mixed) {
+    if (mode == 0)
+      kInterForce<float, 0><<<kIB, kIT, 0, st>>>(s.G, t32, iv, cxp(s.x[s.cur]), s.type, s.df32, s.in, vp);
+    else
+      kInterForce<float, 2><<<kIB, kIT, 0, st>>>(s.G, t32, iv, cxp(s.x[s.cur]), s.type, s.df32, s.in, vp);
+  } else {
+    if (mode == 0)
+      kInterForce<double, 0><<<kIB, kIT, 0, st>>>(s.G, t64, iv, cxp(s.x[s.cur]), s.type, s.df64, s.in, vp);
+    else
+      kInterForce<double, 2><<<kIB, kIT, 0, st>>>(s.G, t64, iv, cxp(s.x[s.cur]), s.type, s.df64, s.in, vp);
+  }
+}
+
+void launchInterFirst(const DevState& s, const VerletParams& vp, hipStream_t st) {
+  kInterFirst<<<kIB, kIT, 0, st>>>(s.in, s.ctl, vp);
+}
+
+void launchInterUpload(const DevState& s, const AtomElement* aos_dev, int64_t n, hipStream_t st) {
+  kInterUpload<<<kIB, kIT, 0, st>>>(s.in, aos_dev, n, s.ctl);
+}
+
+void launchInterDownload(const DevState& s, AtomElement* aos_dev, bool mixed, hipStream_t st) {
+  kInterDownload<<<kIB, kIT, 0, st>>>(s.in, aos_dev, s.ctl, mixed);
+}
+
+void launchInterScaleV(const DevState& s, double factor, hipStream_t st) {
+  kInterScaleV<<<kIB, kIT, 0, st>>>(s.in, s.ctl, factor);
+}
+
+int64_t launchInterThermo(const DevState& s, hipStream_t st) {
+  kInterThermo<<<kIB, kIT, 0, st>>>(s.in, s.ctl, s.partials);
+  return kIB;
+}
+
+int64_t launchInterEnergy(const DevState& s, const Tables<double>& t64, hipStream_t st) {
+  kInterEnergy<<<kIB, kIT, 0, st>>>(s.G, t64, interView(s, false), cxp(s.x[s.cur]), s.type, s.rho, s.in,
+                                    s.partials);
+  return kIB;
+}
+
+}  // namespace misa

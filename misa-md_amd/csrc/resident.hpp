@@ -6,53 +6,48 @@
 
 #include "device_common.hpp"
 #include "host_setup.hpp"
+#include "inter_common.hpp"
 
 namespace misa {
 
 // Inter atoms (atoms that left their lattice site, inter_atom_list.h:30-80).
-// Local list in SoA form with a fixed capacity; ghost images and the
-// nearest-site bucket index are rebuilt every step.
+// The local list is SoA in list order with a fixed capacity; the entries
+// (local atoms first, then their periodic ghost images) and the nearest-site
+// bucket index are rebuilt every step by the decide kernel.
 struct InterDev {
-  int64_t cap = 0;
+  int64_t cap = 0;   // local list capacity
+  int64_t ecap = 0;  // entry capacity (local + ghost images)
+  int64_t kcap = 0;  // sort-key capacity (power of two >= ecap)
   uint64_t* id = nullptr;
   int8_t* type = nullptr;
   double* x[3] = {nullptr, nullptr, nullptr};
   double* v[3] = {nullptr, nullptr, nullptr};
   double* f[3] = {nullptr, nullptr, nullptr};
   double* rho = nullptr;
-  double* df = nullptr;
-  // scratch of the same capacity for list maintenance (order-preserving moves)
+  double* df64 = nullptr;
+  float* df32 = nullptr;
+  // scratch of the same capacity for order-preserving moves
   uint64_t* id2 = nullptr;
   int8_t* type2 = nullptr;
   double* x2[3] = {nullptr, nullptr, nullptr};
   double* v2[3] = {nullptr, nullptr, nullptr};
-  // new inter atoms found by the Verlet kernels (unordered, sorted later)
+  int32_t* tmp = nullptr;  // per local atom: recombination target / group
+  // new inter atoms found by the lattice Verlet kernels (unordered)
   int64_t* stage_key = nullptr;  // reference linear index of the vacated site
   int64_t* stage_slot = nullptr;
   uint64_t* stage_id = nullptr;
   int8_t* stage_type = nullptr;
   double* stage_x[3] = {nullptr, nullptr, nullptr};
   double* stage_v[3] = {nullptr, nullptr, nullptr};
-  // ghost images: source index into the local list, image position
-  int64_t gcap = 0;
-  int32_t* g_src = nullptr;
-  double* g_x[3] = {nullptr, nullptr, nullptr};
-  // bucket index over local + ghost entries: sorted (slot << 24 | entry) keys
+  // entries: local atoms (0..n_inter-1) then ghost images
+  int32_t* e_src = nullptr;
+  double* e_x[3] = {nullptr, nullptr, nullptr};
+  int64_t* e_slot = nullptr;
   uint64_t* keys = nullptr;
-  uint64_t* keys_tmp = nullptr;
-  int32_t* bucket_head = nullptr;  // per slot, first sorted position or -1
-  int64_t* prev_slots = nullptr;   // nearest slots of the previous step's entries (flag clearing)
-};
-
-// Control block in device memory (counters and error flags).
-struct Ctl {
-  int64_t n_inter;      // local inter atoms
-  int64_t n_ghost;      // ghost images
-  int64_t n_stage;      // new inter atoms staged by the Verlet kernels
-  int64_t n_prev;       // entries recorded in prev_slots
-  int64_t err_flags;    // bit 0: inter capacity, bit 1: ghost capacity, bit 2: non-finite, bit 3: species
-  int64_t n_keys;
-  int64_t pad[2];
+  int32_t* head = nullptr;        // per plane slot
+  int64_t* prev_slot = nullptr;   // bucket slots of the previous index (reset)
+  int32_t* site_claim = nullptr;  // per plane slot, INT_MAX when idle
+  FullPlaneOffsets* full = nullptr;
 };
 
 struct DevState {
@@ -91,6 +86,25 @@ int64_t launchThermo(const DevState& s, hipStream_t st);
 // Partial sums per block: [e_embed, e_pair, virial] (3 per block)
 int64_t launchEnergy(const DevState& s, const LatOffsets& o, const Tables<double>& t64, hipStream_t st);
 void launchScaleV(const DevState& s, double factor, hipStream_t st);
+// Inter-atom work (kernels_inter.hip).
+// decide mode 0: the step's ownership pass (append new inter atoms, vacancy
+// recombination, migration, ghost images, index, flags); mode 1: migration,
+// ghost images, index and flags only (first force, PKA step, upload).
+void launchDecide(const DevState& s, int mode, hipStream_t st);
+InterView interView(const DevState& s, bool mixed);
+void launchInterRho(const DevState& s, const Tables<double>& t64, const Tables<float>& t32, bool mixed,
+                    hipStream_t st);
+// mode 0: f stored, v += c f (secondstep); mode 2: f stored only
+void launchInterForce(const DevState& s, const Tables<double>& t64, const Tables<float>& t32, bool mixed, int mode,
+                      const VerletParams& vp, hipStream_t st);
+void launchInterFirst(const DevState& s, const VerletParams& vp, hipStream_t st);
+void launchInterUpload(const DevState& s, const AtomElement* aos_dev, int64_t n, hipStream_t st);
+void launchInterDownload(const DevState& s, AtomElement* aos_dev, bool mixed, hipStream_t st);
+void launchInterScaleV(const DevState& s, double factor, hipStream_t st);
+// partials per block: [mvv, fx, fy, fz, vmax2] (5 per block), 64 blocks
+int64_t launchInterThermo(const DevState& s, hipStream_t st);
+// partials per block: [e_embed, e_pair, virial] (3 per block), 64 blocks
+int64_t launchInterEnergy(const DevState& s, const Tables<double>& t64, hipStream_t st);
 void launchSetv(const DevState& s, int64_t slot, double dv0, double dv1, double dv2, hipStream_t st);
 
 }  // namespace misa

@@ -213,6 +213,9 @@ class EamDevice:
         r = (ctypes.c_int32 * 3)(*ratio)
         self._check(self.lib.misa_build_world(self.h, seed, tset, r, alloy_seed), "misa_build_world")
 
+    def set_inter_capacity(self, cap):
+        self._check(self.lib.misa_set_inter_capacity(self.h, int(cap)), "misa_set_inter_capacity")
+
     def set_precision(self, mode):
         self._check(self.lib.misa_set_precision(self.h, mode), "misa_set_precision")
 
