@@ -34,6 +34,9 @@ HBM_PEAK_GBS = 8000.0
 FORCE_BYTES = {"fp64": 105, "mixed": 101}
 RHO_BYTES = {"fp64": 33, "mixed": 29}  # read x, type; write F'
 STEP_BYTES = {"fp64": 138, "mixed": 130}
+# HBM traffic of one force-pass launch, from the committed rocprofv3 counter
+# summary of this workload (FETCH_SIZE + WRITE_SIZE, calibrated on a copy)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "kforce_mixed_fecuni_200c_traffic.txt")
 
 
 def log(*a):
@@ -41,19 +44,30 @@ def log(*a):
 
 
 def run_precision(dev, mode, steps, warmup):
+    """Warm-up steps, then `steps` timed steps; HIP events around every
+    launch on the library's stream split the timed region by kernel."""
     dev.set_precision(mode)
     dev.compute_forces()
     if warmup:
         dev.step(warmup)
-    t0 = time.perf_counter()
-    dev.step(steps)
-    secs = time.perf_counter() - t0
-    # per-kernel device time with HIP events on the library's stream
     dev.set_profiling(True)
-    dev.step(max(5, min(steps, 20)))
+    t0 = time.perf_counter()
+    dev.step(steps)  # returns after the device has finished
+    secs = time.perf_counter() - t0
     tim = dev.last_timings()
     dev.set_profiling(False)
     return secs, tim
+
+
+def traffic_bytes():
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def force_error(dev):
@@ -82,7 +96,7 @@ def cpu_baseline(setfl):
     exe = oracle_py.BASELINE_BIN
     if not os.path.exists(exe):
         return None
-    cells, steps = 40, 10
+    cells, steps = 50, 20
     out = subprocess.run([exe, setfl, str(cells), str(cells), str(cells), "985", "10", "5", "300", "2020",
                           str(steps)], check=True, capture_output=True, text=True, timeout=600)
     r = json.loads(out.stdout.strip().splitlines()[-1])
@@ -131,6 +145,19 @@ def main():
     avg_rho_ms = t32["rho_df_ms"] / max(1, t32["launches"][0])
     achieved = natoms * FORCE_BYTES["mixed"] / (avg_force_ms * 1e-3) / 1e9
     avg_force64 = t64["force_ms"] / max(1, t64["launches"][1])
+    avg_rho64 = t64["rho_df_ms"] / max(1, t64["launches"][0])
+
+    def gbs(nbytes, ms):
+        return natoms * nbytes / (ms * 1e-3) / 1e9
+
+    kernels = {
+        "mixed": {"kForceM_ms": avg_force_ms, "kForceM_GBs": achieved,
+                  "kRhoDfM_ms": avg_rho_ms, "kRhoDfM_GBs": gbs(RHO_BYTES["mixed"], avg_rho_ms),
+                  "other_ms_per_step": t32["other_ms"] / args.steps},
+        "fp64": {"kForce_ms": avg_force64, "kForce_GBs": gbs(FORCE_BYTES["fp64"], avg_force64),
+                 "kRhoDf_ms": avg_rho64, "kRhoDf_GBs": gbs(RHO_BYTES["fp64"], avg_rho64),
+                 "other_ms_per_step": t64["other_ms"] / args.steps},
+    }
     res = {
         "metric": METRIC,
         "value": rate32,
@@ -142,19 +169,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-mixed (f32 splines/pair terms, f64 positions/Verlet/sums)",
+        "dtype": "f32 (mixed: f32 splines/pair terms, f64 positions/Verlet/sums)",
         "data": "synthetic",
         "config": {"workload": "BCC FeCuNi %d^3 cells (%d atoms), Fe:Cu:Ni 985:10:5, 300 K, dt 1 fs, "
                                "synthetic 3-element setfl Nr=Nrho=5000 rc 5.6 A" % (n, natoms),
                    "atoms": natoms, "parallelism": "single GPU"},
-        "fp64": {"value": rate64, "unit": "atom-steps/s", "ms_per_step": s64 * 1e3 / args.steps,
-                 "force_kernel_ms": avg_force64, "rho_kernel_ms": t64["rho_df_ms"] / max(1, t64["launches"][0])},
+        "fp64": {"value": rate64, "unit": "atom-steps/s", "ms_per_step": s64 * 1e3 / args.steps},
+        "kernels": kernels,
         "mixed_over_fp64": rate32 / rate64,
         "force_err_mixed_vs_fp64": {"max": fmax, "rms": frms, "rms_force_eV_per_A": f_scale},
-        "roofline": {"bound": "hbm", "kernel": "kForce (force + Verlet pass, mixed)",
+        "roofline": {"bound": "hbm", "kernel": "kForceM (force + Verlet pass, mixed)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes_per_atom": FORCE_BYTES["mixed"],
-                     "avg_launch_ms": avg_force_ms, "rho_kernel_ms": avg_rho_ms},
+                     "traffic": traffic_bytes(), "algorithmic_bytes_per_launch": natoms * FORCE_BYTES["mixed"],
+                     "algorithmic_bytes_per_atom": FORCE_BYTES["mixed"], "avg_launch_ms": avg_force_ms},
         "step_roofline_frac": rate32 * STEP_BYTES["mixed"] / (HBM_PEAK_GBS * 1e9),
         "thermo": {"temperature": th["temperature"], "e_potential_per_atom": th["e_potential"] / natoms},
         "cpu_baseline": cpu,

@@ -26,6 +26,7 @@ struct misa_handle {
   LatOffsets lat{};
   FullOffsets full{};
   FullPlaneOffsets fullPlane{};
+  float4 dlHost[2 * kMaxLatOffsets]{};  // lattice vectors of the pruned offsets (mixed records)
   void* interBlock = nullptr;  // one allocation carved into the inter-atom arrays
   AtomElement* interAos = nullptr;
   int64_t interAosCap = 0;
@@ -90,7 +91,7 @@ int buildPlaneOffsets(misa_handle* h) {
   for (int p = 0; p < 2; p++) {
     const std::vector<int64_t>& L = p == 0 ? h->ref.even : h->ref.odd;
     if (static_cast<int>(L.size()) > kMaxFullOffsets) return fail(h, MISA_E_ARG, "offset list too long");
-    int nl = 0;
+    int nl = 0, nw = 0;
     h->full.n[p] = static_cast<int>(L.size());
     for (size_t k = 0; k < L.size(); k++) {
       PlaneOffset po;
@@ -103,20 +104,34 @@ int buildPlaneOffsets(misa_handle* h) {
         h->fullPlane.off[p][k] =
             static_cast<int32_t>(po.dp * NX * NY * NZ + (static_cast<int64_t>(po.dz) * NY + po.dy) * NX + po.dcx);
       }
-      if (po.len < keep) {
+      if (po.len >= keep) {
+        const int64_t NX = b.n[0] + 2 * b.g, NY = b.n[1] + 2 * b.g, NZ = b.n[2] + 2 * b.g;
+        h->fullPlane.woff[p][nw++] =
+            static_cast<int32_t>(po.dp * NX * NY * NZ + (static_cast<int64_t>(po.dz) * NY + po.dy) * NX + po.dcx);
+      } else {
         if (nl >= kMaxLatOffsets) return fail(h, MISA_E_ARG, "pruned offset list too long");
         const int64_t NX = b.n[0] + 2 * b.g, NY = b.n[1] + 2 * b.g, NZ = b.n[2] + 2 * b.g;
         const int64_t plane = NX * NY * NZ;
         const int64_t d = po.dp * plane + (static_cast<int64_t>(po.dz) * NY + po.dy) * NX + po.dcx;
+        // lattice vector from the centre's site to the neighbour's site
+        const double hdp = 0.5 * po.dp;
+        h->dlHost[p * kMaxLatOffsets + nl] =
+            make_float4(static_cast<float>((po.dcx + hdp) * b.a), static_cast<float>((po.dy + hdp) * b.a),
+                        static_cast<float>((po.dz + hdp) * b.a), 0.0f);
         h->lat.off[p][nl++] = static_cast<int32_t>(d);
       }
     }
     h->lat.n[p] = nl;
     h->fullPlane.n[p] = static_cast<int>(L.size());
+    h->fullPlane.nw[p] = nw;
   }
   if (h->fullPlane.n[0] != h->fullPlane.n[1]) return fail(h, MISA_E_ARG, "even and odd offset lists differ in length");
   if (h->s.in.full) {
     hipError_t e = hipMemcpy(h->s.in.full, &h->fullPlane, sizeof(FullPlaneOffsets), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(h, MISA_E_HIP, std::string("hipMemcpy offsets: ") + hipGetErrorString(e));
+  }
+  if (h->s.dl) {
+    hipError_t e = hipMemcpy(h->s.dl, h->dlHost, sizeof(h->dlHost), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail(h, MISA_E_HIP, std::string("hipMemcpy offsets: ") + hipGetErrorString(e));
   }
   // non-negative form for the buffer loads: sh = off - lo, lo = most negative delta
@@ -192,6 +207,8 @@ int allocInter(misa_handle* h, int64_t cap) {
   add(&in.df64, cap, 8);
   add(&in.df32, cap, 4);
   add(&in.tmp, cap, 4);
+  add(&in.far_new, cap, 8);
+  add(&in.far_prev, cap, 8);
   add(&in.stage_key, cap, 8);
   add(&in.stage_slot, cap, 8);
   add(&in.stage_id, cap, 8);
@@ -308,12 +325,15 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
   }
   if (rc == MISA_OK) rc |= devAlloc(h, &s.rho, slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.df64, slots);
-  if (rc == MISA_OK) rc |= devAlloc(h, &s.df32, slots);
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.rec[0], slots);
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.rec[1], slots);
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.dl, 2 * kMaxLatOffsets);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.type, slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.id, slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.ctl, 1);
   const int64_t blocks = 16 * G.W.bpx;
-  s.npartials = 8 * blocks;
+  // room for the site kernels' 7 partials per block and the inter kernels' 5
+  s.npartials = std::max<int64_t>(8 * blocks, 8 * kInterBlocks);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.partials, static_cast<size_t>(s.npartials));
   if (rc == MISA_OK) rc |= allocInter(h, 65536);
   if (rc != MISA_OK) {
@@ -328,7 +348,8 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
   }
   hipMemset(s.rho, 0, slots * 8);
   hipMemset(s.df64, 0, slots * 8);
-  hipMemset(s.df32, 0, slots * 4);
+  hipMemset(s.rec[0], 0, slots * sizeof(float4));
+  hipMemset(s.rec[1], 0, slots * sizeof(float4));
   hipMemset(s.type, 0xff, slots);
   hipMemset(s.id, 0, slots * 8);
   hipMemset(s.ctl, 0, sizeof(Ctl));
@@ -683,7 +704,10 @@ int misa_compute_forces(misa_handle* h) {
   const VerletParams vp = verletOf(h->dt);
   const bool prof = h->profile;
   h->profile = false;
-  launchDecide(h->s, 1, h->stream);
+  const bool mixed = h->precision == MISA_PREC_MIXED;
+  if (mixed) launchPackRec(h->s, h->s.cur, h->stream);
+  launchFarScan(h->s, h->stream);
+  launchDecide(h->s, 1, mixed, h->stream);
   forcePass(h, 2, 2, true, vp);
   h->profile = prof;
   HIPCHECK(h, hipGetLastError());
@@ -699,6 +723,7 @@ int misa_step(misa_handle* h, int64_t nsteps) {
   if (nsteps == 0) return MISA_OK;
   HIPCHECK(h, hipSetDevice(h->device));
   const VerletParams vp = verletOf(h->dt);
+  const bool mixed = h->precision == MISA_PREC_MIXED;
   for (auto e : h->events) hipEventDestroy(e);
   h->events.clear();
   h->event_cls.clear();
@@ -709,13 +734,13 @@ int misa_step(misa_handle* h, int64_t nsteps) {
   // firststep + decide of the first step (simulation.cpp:168-176)
   {
     Timer t(h, 2);
-    launchFirst(h->s, vp, h->stream);
+    launchFirst(h->s, mixed, vp, h->stream);
     launchInterFirst(h->s, vp, h->stream);
   }
   h->s.cur = 1 - h->s.cur;
   {
     Timer t(h, 2);
-    launchDecide(h->s, 0, h->stream);
+    launchDecide(h->s, 0, mixed, h->stream);
   }
   for (int64_t i = 0; i < nsteps; i++) {
     const bool last = i == nsteps - 1;
@@ -729,7 +754,7 @@ int misa_step(misa_handle* h, int64_t nsteps) {
       h->s.cur = 1 - h->s.cur;
       {
         Timer t(h, 2);
-        launchDecide(h->s, 0, h->stream);
+        launchDecide(h->s, 0, mixed, h->stream);
       }
     }
   }

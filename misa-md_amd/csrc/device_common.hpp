@@ -15,9 +15,11 @@
 // host_setup.cpp), so the pairs visited are the reference's pairs.
 //
 // State is SoA: x (two ping-pong buffers per component), v, f, df, rho, a
-// type byte and the atom id.  The type byte holds the species in bits 0-1,
-// bit 6 marks a site with an inter atom in its 229-site neighbourhood, and a
-// negative value marks a vacancy (atom_type::INVALID).
+// type byte and the atom id; mixed precision adds 16-byte site records
+// (site_common.hpp).  The type byte holds the species in bits 0-1, bit 6
+// marks a site with an inter atom in its 229-site neighbourhood, bit 5 a site
+// near a lattice atom far from its site (kWide), and a negative value marks a
+// vacancy (atom_type::INVALID).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -30,6 +32,11 @@ constexpr int kMaxFullOffsets = 256;  // reference full list per parity
 constexpr int kKnot = 8;              // values per knot record: c0 c1 c2 _ c3 c4 c5 c6
 constexpr int8_t kVacant = -1;
 constexpr int8_t kNearInter = 0x40;
+// A site within the full neighbourhood of a lattice atom that sits farther
+// than 0.2a from its site (an inter atom that has just refilled a vacancy
+// can sit anywhere in that site's Wigner-Seitz cell): its sums also visit
+// the full-list offsets the pruned list leaves out.
+constexpr int8_t kWide = 0x20;
 
 // Site traversal of the resident kernels: each XCD takes a z-slab of both
 // parity planes and walks it in y-tiles of `ty` rows (z inner), so the
@@ -59,9 +66,10 @@ struct Ctl {
   int64_t n_ghost;      // ghost images
   int64_t n_stage;      // new inter atoms staged by the Verlet kernels
   int64_t n_prev;       // entries recorded in prev_slots
-  int64_t err_flags;    // bit 0: inter capacity, bit 1: ghost capacity, bit 2: non-finite, bit 3: species
-  int64_t n_keys;
-  int64_t pad[2];
+  int64_t err_flags;    // bit 0: inter capacity, bit 1: ghost capacity, bit 2: non-finite, bit 3: left the box
+  int64_t n_far;        // lattice atoms farther than 0.2a from their site, found for the next index
+  int64_t n_far_prev;   // the same, as flagged by the last decide
+  int64_t pad;
 };
 
 // Pruned lattice-lattice neighbour list: slot deltas per centre parity, and

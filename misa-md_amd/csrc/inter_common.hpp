@@ -17,6 +17,9 @@ namespace misa {
 struct FullPlaneOffsets {
   int n[2];
   int32_t off[2][kMaxFullOffsets];
+  // the full-list offsets missing from the pruned lattice list (kWide sites)
+  int nw[2];
+  int32_t woff[2][kMaxFullOffsets];
 };
 
 // What the site kernels need to see the inter atoms.

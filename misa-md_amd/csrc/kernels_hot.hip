@@ -11,10 +11,12 @@
 // One thread per site.  A full list (each lane owns its site's sums) replaces
 // the reference's half list with ghost partials and reverse halos: no atomics,
 // deterministic sums.  The list keeps the reference offsets whose ideal
-// separation is below rc + 0.4a (+0.01a): a lattice atom stays within 0.2a of
-// its site (decide() turns it into an inter atom otherwise), so every dropped
-// offset is beyond the cutoff for every lattice pair and the visited pair set
-// is the reference's.
+// separation is below rc + 0.4a (+0.01a): a lattice atom normally stays within
+// 0.2a of its site (decide() turns it into an inter atom otherwise), so every
+// dropped offset is beyond the cutoff for such pairs.  The exception is an
+// inter atom that has just refilled a vacancy anywhere in its Wigner-Seitz
+// cell: the sites within its full neighbourhood carry kWide and also visit
+// the dropped offsets, so the visited pair set is always the reference's.
 //
 // Memory access: every lane of a wavefront reads neighbour k at the same slot
 // delta, so x, type and F' are fetched with buffer loads whose per-lane
@@ -175,6 +177,16 @@ __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T>
       bucket(s);
       for (int q = 0; q < full.n[p]; q++) bucket(s + full.off[p][q]);
     }
+    if (ti & kWide) {  // full-list offsets beyond the pruned list, near a far lattice atom
+      const FullPlaneOffsets& full = *iv.full;
+      for (int q = 0; q < full.nw[p]; q++) {
+        const int64_t j = s + full.woff[p][q];
+        const int8_t tj = type[j];
+        const double d0 = xi0 - x.c[0][j], d1 = xi1 - x.c[1][j], d2 = xi2 - x.c[2][j];
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        if (!isVacant(tj) && r2 < G.rc2) acc.add(rhoPair(tb, species(tj), r2));
+      }
+    }
     rho = acc.value();
     dfi = dEmbed(tb, species(ti), static_cast<T>(rho));
   }
@@ -314,6 +326,16 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
     bucket(s);
     for (int q = 0; q < full.n[p]; q++) bucket(s + full.off[p][q]);
   }
+  if (ti & kWide) {  // full-list offsets beyond the pruned list, near a far lattice atom
+    const FullPlaneOffsets& full = *iv.full;
+    for (int q = 0; q < full.nw[p]; q++) {
+      const int64_t j = s + full.woff[p][q];
+      const int8_t tj = type[j];
+      const double e0 = xi0 - x.c[0][j], e1 = xi1 - x.c[1][j], e2 = xi2 - x.c[2][j];
+      const double r2 = e0 * e0 + e1 * e1 + e2 * e2;
+      if (!isVacant(tj) && r2 < G.rc2) acc.add(e0, e1, e2, pairForce(tb, ei, species(tj), r2, dfi, df[j]));
+    }
+  }
   double f0, f1, f2;
   sum3Value<ACC>(acc, f0, f1, f2);
   const double c = vp.c[ei];
@@ -358,6 +380,268 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
   });
 }
 
+// ---------------------------------------------------------------------------
+// Mixed precision on site records: one 16-byte load per neighbour.
+//   x_i - x_j = (u_i - u_j) - (site_j - site_i), the lattice vector of the
+//   offset coming from a per-offset table in scalar registers.  The FP32
+//   displacement differs from the FP64 one by a few 1e-7 relative, far inside
+//   the mixed path's error bar, and the pair set differs from FP64's only for
+//   pairs within that distance of the cutoff, where the tabulated functions
+//   and their slopes vanish.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float4 ldRec4(Rsrc r, uint32_t v, uint32_t s) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, 0));
+}
+
+template <int ACC, int U>
+__global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<float> tb, CXPtr x,
+                                                  const int8_t* __restrict__ type, float4* rec,
+                                                  const float4* __restrict__ dl, double* __restrict__ rho_out,
+                                                  InterView iv) {
+  int64_t s;
+  int p, cx, cy, cz;
+  if (!siteOfThread(G, s, p, cx, cy, cz)) return;
+  const int8_t ti = type[s];
+  float dfi = 0.0f;
+  double rho = 0.0;
+  if (!isVacant(ti)) {
+    const float4 me = rec[s];
+    const Rsrc rr = rsrcBelow(rec, O.lo);
+    const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoV));
+    const uint32_t o16 = static_cast<uint32_t>(s) * 16u;
+    const uint32_t* sh = O.sh[p];
+    const float4* D = dl + p * kMaxLatOffsets;
+    const float rc2 = static_cast<float>(G.rc2);
+    const int nk = O.n[p];
+    Sum1<ACC> acc;
+    auto group = [&](auto UC, int k0) {
+      constexpr int V = decltype(UC)::value;
+      float4 nb[V];
+#pragma unroll
+      for (int u = 0; u < V; ++u) nb[u] = ldRec4(rr, o16, sh[k0 + u] * 16u);
+      bool in[V];
+      float pp[V];
+      float cr[V][kRhoRec];
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        const float4 d = D[k0 + u];
+        const float d0 = (me.x - nb[u].x) - d.x, d1 = (me.y - nb[u].y) - d.y, d2 = (me.z - nb[u].z) - d.z;
+        const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        in[u] = r2 < rc2;
+        if (in[u]) {
+          const int m = knotOf(r2 * __builtin_amdgcn_rsqf(r2), tb.rdr, tb.nr, false, pp[u]);
+          const uint32_t off = static_cast<uint32_t>(recSpecies(nb[u].x) * tb.nr + m) *
+                               static_cast<uint32_t>(kRhoRec * sizeof(float));
+          ldRec<float, kRhoRec>(rtab, off, cr[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < V; ++u)
+        if (in[u]) acc.add(((cr[u][0] * pp[u] + cr[u][1]) * pp[u] + cr[u][2]) * pp[u] + cr[u][3]);
+    };
+    int k = 0;
+    for (; k + U <= nk; k += U) group(std::integral_constant<int, U>{}, k);
+    for (; k < nk; ++k) group(std::integral_constant<int, 1>{}, k);
+    if (ti & kNearInter) {  // inter atoms whose nearest site is within the full list (atom.cpp:224-240)
+      const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
+      const int ne = static_cast<int>(iv.ctl->n_inter + iv.ctl->n_ghost);
+      const FullPlaneOffsets& full = *iv.full;
+      auto bucket = [&](int64_t q) {
+        forBucket(iv, q, ne, [&](int e) {
+          const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
+          const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+          if (r2 < G.rc2) acc.add(rhoPair(tb, species(iv.type[iv.e_src[e]]), r2));
+        });
+      };
+      bucket(s);
+      for (int q = 0; q < full.n[p]; q++) bucket(s + full.off[p][q]);
+    }
+    if (ti & kWide) {  // full-list offsets beyond the pruned list, near a far lattice atom
+      const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
+      const FullPlaneOffsets& full = *iv.full;
+      for (int q = 0; q < full.nw[p]; q++) {
+        const int64_t j = s + full.woff[p][q];
+        const int8_t tj = type[j];
+        const double d0 = xi0 - x.c[0][j], d1 = xi1 - x.c[1][j], d2 = xi2 - x.c[2][j];
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        if (!isVacant(tj) && r2 < G.rc2) acc.add(rhoPair(tb, species(tj), r2));
+      }
+    }
+    rho = acc.value();
+    dfi = dEmbed(tb, species(ti), static_cast<float>(rho));
+  }
+  float* w = reinterpret_cast<float*>(rec) + 3;
+  w[4 * s] = dfi;
+  if (rho_out) rho_out[s] = rho;
+  forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) {
+    w[4 * q] = dfi;
+    if (rho_out) rho_out[q] = rho;
+  });
+}
+
+template <int ACC, int MODE, int U>
+__global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<float> tb, CXPtr x, XPtr xn,
+                                                  const int8_t* __restrict__ type, const float4* __restrict__ rec,
+                                                  float4* __restrict__ recn, const float4* __restrict__ dl,
+                                                  XPtr v, XPtr f, const uint64_t* __restrict__ id, VerletParams vp,
+                                                  StageArgs st, Ctl* ctl, InterView iv) {
+  int64_t s;
+  int p, cx, cy, cz;
+  if (!siteOfThread(G, s, p, cx, cy, cz)) return;
+  const int8_t ti = type[s];
+  const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
+  if (isVacant(ti)) {  // vacancy: keeps its stale position (atom.cpp:43-47)
+    if (MODE != 1) {   // clearForce left 0 there and latForce skips it (atom.cpp:86-100, 311-358)
+      f.c[0][s] = 0.0;
+      f.c[1][s] = 0.0;
+      f.c[2][s] = 0.0;
+    } else {
+      xn.c[0][s] = xi0;
+      xn.c[1][s] = xi1;
+      xn.c[2][s] = xi2;
+      recn[s] = vacantRec();
+      forEachImage(G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
+        xn.c[0][q] = xi0 + sx * G.L0;
+        xn.c[1][q] = xi1 + sy * G.L1;
+        xn.c[2][q] = xi2 + sz * G.L2;
+        recn[q] = vacantRec();
+      });
+    }
+    return;
+  }
+  const int ei = species(ti);
+  const float4 me = rec[s];
+  const float dfi = me.w;
+  const Rsrc rr = rsrcBelow(rec, O.lo);
+  const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairF));
+  const Rsrc rtabR = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoV));
+  const uint32_t o16 = static_cast<uint32_t>(s) * 16u;
+  const uint32_t pairBase = static_cast<uint32_t>(ei * 3) * static_cast<uint32_t>(tb.nr);
+  const uint32_t* sh = O.sh[p];
+  const float4* D = dl + p * kMaxLatOffsets;
+  const float rc2 = static_cast<float>(G.rc2);
+  const int nk = O.n[p];
+  Sum3<ACC> acc;
+  auto group = [&](auto UC, int k0) {
+    constexpr int V = decltype(UC)::value;
+    float4 nb[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) nb[u] = ldRec4(rr, o16, sh[k0 + u] * 16u);
+    bool in[V];
+    float d0[V], d1[V], d2[V], pp[V], ri[V];
+    float rec8[V][kPairRec], reci[V][kRhoRec];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      const float4 d = D[k0 + u];
+      d0[u] = (me.x - nb[u].x) - d.x;
+      d1[u] = (me.y - nb[u].y) - d.y;
+      d2[u] = (me.z - nb[u].z) - d.z;
+      const float r2 = d0[u] * d0[u] + d1[u] * d1[u] + d2[u] * d2[u];
+      in[u] = r2 < rc2;
+      if (in[u]) {
+        ri[u] = __builtin_amdgcn_rsqf(r2);
+        const int m = knotOf(r2 * ri[u], tb.rdr, tb.nr, false, pp[u]);
+        const int ej = recSpecies(nb[u].x);
+        const uint32_t off = (pairBase + static_cast<uint32_t>(ej * tb.nr + m)) *
+                             static_cast<uint32_t>(kPairRec * sizeof(float));
+        ldRec<float, kPairRec>(rtab, off, rec8[u]);
+        if (ej != ei) {  // rho_ei differs from rho_ej only for unlike pairs
+          const uint32_t offi = static_cast<uint32_t>(ei * tb.nr + m) * static_cast<uint32_t>(kRhoRec * sizeof(float));
+          ldRec<float, kRhoRec>(rtabR, offi, reci[u]);
+        } else {
+#pragma unroll
+          for (int q = 0; q < kRhoRec; q++) reci[u][q] = rec8[u][4 + q];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      if (in[u]) {
+        const float q = pp[u], rinv = ri[u];
+        const float* z = rec8[u];
+        const float* rj = rec8[u] + 4;
+        const float* rI = reci[u];
+        const float z2 = ((z[0] * q + z[1]) * q + z[2]) * q + z[3];
+        const float z2p = ((3.0f * z[0] * q + 2.0f * z[1]) * q + z[2]) * tb.rdr;
+        const float rhojp = ((3.0f * rj[0] * q + 2.0f * rj[1]) * q + rj[2]) * tb.rdr;
+        const float rhoip = ((3.0f * rI[0] * q + 2.0f * rI[1]) * q + rI[2]) * tb.rdr;
+        const float phi = z2 * rinv;
+        const float phip = z2p * rinv - phi * rinv;
+        const float psip = dfi * rhojp + nb[u].w * rhoip + phip;
+        acc.add(d0[u], d1[u], d2[u], -psip * rinv);
+      }
+    }
+  };
+  int k = 0;
+  for (; k + U <= nk; k += U) group(std::integral_constant<int, U>{}, k);
+  for (; k < nk; ++k) group(std::integral_constant<int, 1>{}, k);
+  if (ti & kNearInter) {  // forces from inter atoms on lattice neighbours (atom.cpp:397-422)
+    const int ne = static_cast<int>(iv.ctl->n_inter + iv.ctl->n_ghost);
+    const FullPlaneOffsets& full = *iv.full;
+    auto bucket = [&](int64_t q) {
+      forBucket(iv, q, ne, [&](int e) {
+        const int src = iv.e_src[e];
+        const double e0 = xi0 - iv.e_x[0][e], e1 = xi1 - iv.e_x[1][e], e2 = xi2 - iv.e_x[2][e];
+        const double r2 = e0 * e0 + e1 * e1 + e2 * e2;
+        if (r2 < G.rc2) acc.add(e0, e1, e2, pairForce(tb, ei, species(iv.type[src]), r2, dfi, iv.df32[src]));
+      });
+    };
+    bucket(s);
+    for (int q = 0; q < full.n[p]; q++) bucket(s + full.off[p][q]);
+  }
+  if (ti & kWide) {  // full-list offsets beyond the pruned list, near a far lattice atom
+    const FullPlaneOffsets& full = *iv.full;
+    for (int q = 0; q < full.nw[p]; q++) {
+      const int64_t j = s + full.woff[p][q];
+      const int8_t tj = type[j];
+      const double e0 = xi0 - x.c[0][j], e1 = xi1 - x.c[1][j], e2 = xi2 - x.c[2][j];
+      const double r2 = e0 * e0 + e1 * e1 + e2 * e2;
+      if (!isVacant(tj) && r2 < G.rc2) acc.add(e0, e1, e2, pairForce(tb, ei, species(tj), r2, dfi, rec[j].w));
+    }
+  }
+  double f0, f1, f2;
+  sum3Value<ACC>(acc, f0, f1, f2);
+  const double c = vp.c[ei];
+  double v0 = v.c[0][s] + c * f0;
+  double v1 = v.c[1][s] + c * f1;
+  double v2 = v.c[2][s] + c * f2;
+  if (MODE != 1) {
+    f.c[0][s] = f0;
+    f.c[1][s] = f1;
+    f.c[2][s] = f2;
+    if (MODE == 0) {
+      v.c[0][s] = v0;
+      v.c[1][s] = v1;
+      v.c[2][s] = v2;
+    }
+    return;
+  }
+  v0 = v0 + c * f0;
+  v1 = v1 + c * f1;
+  v2 = v2 + c * f2;
+  const double y0 = xi0 + vp.dt * v0;
+  const double y1 = xi1 + vp.dt * v1;
+  const double y2 = xi2 + vp.dt * v2;
+  if (leftSite(G, p, cx, cy, cz, y0, y1, y2)) {
+    stageInter(st, ctl, refIndex(G, p, cx, cy, cz), s, id[s], ti, y0, y1, y2, v0, v1, v2);
+    v0 = v1 = v2 = 0.0;
+  }
+  v.c[0][s] = v0;
+  v.c[1][s] = v1;
+  v.c[2][s] = v2;
+  xn.c[0][s] = y0;
+  xn.c[1][s] = y1;
+  xn.c[2][s] = y2;
+  const float4 r = siteRec(G, p, cx, cy, cz, y0, y1, y2, ei);
+  recn[s] = r;
+  forEachImage(G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
+    xn.c[0][q] = y0 + sx * G.L0;
+    xn.c[1][q] = y1 + sy * G.L1;
+    xn.c[2][q] = y2 + sz * G.L2;
+    recn[q] = r;
+  });
+}
+
 constexpr int kUFloat = 4;
 constexpr int kUDouble = 2;
 
@@ -368,12 +652,13 @@ void launchRhoDf(const DevState& s, const LatOffsets& o, const Tables<double>& t
   double* rho = store_rho ? s.rho : nullptr;
   const unsigned grid = siteGrid(s.G);
   const InterView iv = interView(s, mixed);
+  const CXPtr x = cxp(s.x[s.cur]);
   if (!mixed)
-    kRhoDf<double, 0, 4><<<grid, kBlock, 0, st>>>(s.G, o, t64, cxp(s.x[s.cur]), s.type, s.df64, rho, iv);
+    kRhoDf<double, 0, 4><<<grid, kBlock, 0, st>>>(s.G, o, t64, x, s.type, s.df64, rho, iv);
   else if (acc == 0)
-    kRhoDf<float, 0, kUFloat><<<grid, kBlock, 0, st>>>(s.G, o, t32, cxp(s.x[s.cur]), s.type, s.df32, rho, iv);
+    kRhoDfM<0, kUFloat><<<grid, kBlock, 0, st>>>(s.G, o, t32, x, s.type, s.rec[s.cur], s.dl, rho, iv);
   else
-    kRhoDf<float, 1, kUFloat><<<grid, kBlock, 0, st>>>(s.G, o, t32, cxp(s.x[s.cur]), s.type, s.df32, rho, iv);
+    kRhoDfM<1, kUFloat><<<grid, kBlock, 0, st>>>(s.G, o, t32, x, s.type, s.rec[s.cur], s.dl, rho, iv);
 }
 
 void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t64, const Tables<float>& t32,
@@ -383,22 +668,27 @@ void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t
   const XPtr xn = xp(s.x[1 - s.cur]);
   const StageArgs sa = stageOf(s);
   const InterView iv = interView(s, mixed);
-#define MISA_FORCE(T, A, M, U, TB, DF) \
-  kForce<T, A, M, U><<<grid, kBlock, 0, st>>>(s.G, o, TB, x, xn, s.type, DF, xp(s.v), xp(s.f), s.id, vp, sa, s.ctl, iv)
+#define MISA_FORCE(M) \
+  kForce<double, 0, M, kUDouble><<<grid, kBlock, 0, st>>>(s.G, o, t64, x, xn, s.type, s.df64, xp(s.v), xp(s.f), \
+                                                         s.id, vp, sa, s.ctl, iv)
+#define MISA_FORCE_M(A, M)                                                                                       \
+  kForceM<A, M, kUFloat><<<grid, kBlock, 0, st>>>(s.G, o, t32, x, xn, s.type, s.rec[s.cur], s.rec[1 - s.cur], \
+                                                  s.dl, xp(s.v), xp(s.f), s.id, vp, sa, s.ctl, iv)
   if (!mixed) {
-    if (mode == 0) MISA_FORCE(double, 0, 0, kUDouble, t64, s.df64);
-    else if (mode == 1) MISA_FORCE(double, 0, 1, kUDouble, t64, s.df64);
-    else MISA_FORCE(double, 0, 2, kUDouble, t64, s.df64);
+    if (mode == 0) MISA_FORCE(0);
+    else if (mode == 1) MISA_FORCE(1);
+    else MISA_FORCE(2);
   } else if (acc == 0) {
-    if (mode == 0) MISA_FORCE(float, 0, 0, kUFloat, t32, s.df32);
-    else if (mode == 1) MISA_FORCE(float, 0, 1, kUFloat, t32, s.df32);
-    else MISA_FORCE(float, 0, 2, kUFloat, t32, s.df32);
+    if (mode == 0) MISA_FORCE_M(0, 0);
+    else if (mode == 1) MISA_FORCE_M(0, 1);
+    else MISA_FORCE_M(0, 2);
   } else {
-    if (mode == 0) MISA_FORCE(float, 1, 0, kUFloat, t32, s.df32);
-    else if (mode == 1) MISA_FORCE(float, 1, 1, kUFloat, t32, s.df32);
-    else MISA_FORCE(float, 1, 2, kUFloat, t32, s.df32);
+    if (mode == 0) MISA_FORCE_M(1, 0);
+    else if (mode == 1) MISA_FORCE_M(1, 1);
+    else MISA_FORCE_M(1, 2);
   }
 #undef MISA_FORCE
+#undef MISA_FORCE_M
 }
 
 }  // namespace misa

@@ -37,7 +37,7 @@ using namespace site;
 
 constexpr int kDT = 1024;  // threads of the single-workgroup decide kernel
 constexpr int kIT = 256;   // threads per block of the per-atom kernels
-constexpr int kIB = 64;    // blocks of the per-atom kernels (grid-stride)
+constexpr int kIB = kInterBlocks;  // blocks of the per-atom kernels (grid-stride)
 
 struct DecideArgs {
   Geom G;
@@ -47,6 +47,7 @@ struct DecideArgs {
   double* lv[3];
   int8_t* ltype;
   uint64_t* lid;
+  float4* lrec;  // mixed precision: site records of the current buffer, else null
   InterDev in;
   Ctl* ctl;
 };
@@ -128,10 +129,18 @@ __device__ __forceinline__ bool inForwardRegion(const Geom& G, int dim, int dir,
   return true;
 }
 
+// Type (and, for `pos`, position) of a lattice slot to its periodic images;
+// in mixed precision also the site record: NaN for a vacancy, else the
+// displacement of the (new) atom from its site.
 __device__ __forceinline__ void writeLatticeImages(const DecideArgs& A, int64_t slot, bool pos, int8_t t) {
   int p, cx, cy, cz;
   decodeSlot(A.G, slot, p, cx, cy, cz);
   const double y0 = A.lx[0][slot], y1 = A.lx[1][slot], y2 = A.lx[2][slot];
+  float4 r = vacantRec();
+  if (A.lrec) {
+    if (!isVacant(t)) r = siteRec(A.G, p, cx, cy, cz, y0, y1, y2, species(t));
+    A.lrec[slot] = r;
+  }
   forEachImage(A.G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
     A.ltype[q] = t;
     if (pos) {
@@ -139,7 +148,17 @@ __device__ __forceinline__ void writeLatticeImages(const DecideArgs& A, int64_t 
       A.lx[1][q] = y1 + sy * A.G.L1;
       A.lx[2][q] = y2 + sz * A.G.L2;
     }
+    if (A.lrec) A.lrec[q] = r;
   });
+}
+
+// fn(slot) for a sub-box slot and every periodic image of it in the ghost layer.
+template <typename Fn>
+__device__ __forceinline__ void forSlotAndImages(const Geom& G, int64_t slot, Fn fn) {
+  int p, cx, cy, cz;
+  decodeSlot(G, slot, p, cx, cy, cz);
+  fn(slot);
+  forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) { fn(q); });
 }
 
 __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
@@ -148,6 +167,9 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
   const int tid = threadIdx.x;
   int nl = static_cast<int>(A.ctl->n_inter);
   const int nprev = static_cast<int>(A.ctl->n_prev);
+  const int nfarPrev = static_cast<int>(A.ctl->n_far_prev);
+  __shared__ int sFar;  // far lattice atoms for this index (kFarScan's in mode 1)
+  if (tid == 0) sFar = static_cast<int>(min(A.ctl->n_far, in.cap));
   __syncthreads();
   unsigned long long* err = reinterpret_cast<unsigned long long*>(&A.ctl->err_flags);
 
@@ -201,13 +223,22 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
           A.lv[k][t] = in.v[k][i];
         }
         writeLatticeImages(A, t, true, in.type[i]);
-        in.tmp[i] = -2;  // removed
+        in.tmp[i] = -3 - t;  // removed; remembers its site for the claim reset
+        int p, cx, cy, cz;
+        decodeSlot(G, t, p, cx, cy, cz);
+        if (leftSite(G, p, cx, cy, cz, in.x[0][i], in.x[1][i], in.x[2][i])) {
+          const int q = atomicAdd(&sFar, 1);
+          if (q < in.cap) in.far_new[q] = t;
+          else setFlag(err, 1ull);
+        }
       }
     }
     __syncthreads();
+    // release every claim (winners' and losers') for the next step
     for (int i = tid; i < nl; i += kDT) {
       const int t = in.tmp[i];
       if (t >= 0) in.site_claim[t] = INT_MAX;
+      else if (t <= -3) in.site_claim[-3 - t] = INT_MAX;
     }
     __syncthreads();
 
@@ -215,7 +246,7 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
     int kept = 0;
     for (int base = 0; base < nl; base += kDT) {
       const int i = base + tid;
-      const int keep = (i < nl && in.tmp[i] != -2) ? 1 : 0;
+      const int keep = (i < nl && in.tmp[i] > -3) ? 1 : 0;
       int tot;
       const int ex = blockScan(keep, tot);
       if (keep) toScratch(in, i, kept + ex, -1, 0.0);
@@ -313,6 +344,23 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
     }
   }
   __syncthreads();
+  // the same for the wide-list flags around the previous far lattice atoms
+  // and their periodic images
+  for (int q = tid; q < nfarPrev * nfull; q += kDT) {
+    const int e = q / nfull;
+    const int k = q % nfull - 1;
+    forSlotAndImages(G, in.far_prev[e], [&](int64_t slot) {
+      const int par = slot >= G.plane ? 1 : 0;
+      if (k >= in.full->n[par]) return;
+      const int64_t t = k < 0 ? slot : slot + in.full->off[par][k];
+      if (t >= 0 && t < 2 * G.plane && isSubBoxSlot(G, t)) {
+        const int8_t v = A.ltype[t];
+        if (!isVacant(v) && (v & kWide)) A.ltype[t] = static_cast<int8_t>(v & ~kWide);
+      }
+    });
+  }
+  __syncthreads();
+  const int nfar = min(sFar, static_cast<int>(in.cap));
   // keys = (slot << 24 | entry), sorted: buckets in slot order, insertion order within
   int P = 1;
   while (P < ne) P <<= 1;
@@ -366,12 +414,46 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
     }
   }
   __syncthreads();
+  // P8: the sites whose sums must visit the whole full list: the full
+  // neighbourhood of every lattice atom farther than 0.2a from its site, and
+  // of each of its periodic images
+  for (int q = tid; q < nfar * nfull; q += kDT) {
+    const int e = q / nfull;
+    const int k = q % nfull - 1;
+    forSlotAndImages(G, in.far_new[e], [&](int64_t slot) {
+      const int par = slot >= G.plane ? 1 : 0;
+      if (k >= in.full->n[par]) return;
+      const int64_t t = k < 0 ? slot : slot + in.full->off[par][k];
+      if (t >= 0 && t < 2 * G.plane && isSubBoxSlot(G, t)) {
+        const int8_t v = A.ltype[t];
+        if (!isVacant(v) && !(v & kWide)) A.ltype[t] = static_cast<int8_t>(v | kWide);
+      }
+    });
+  }
+  for (int q = tid; q < nfar; q += kDT) in.far_prev[q] = in.far_new[q];
+  __syncthreads();
   if (tid == 0) {
     A.ctl->n_inter = nl;
     A.ctl->n_ghost = ne - nl;
     A.ctl->n_stage = 0;
     A.ctl->n_prev = ne;
+    A.ctl->n_far_prev = nfar;
+    A.ctl->n_far = 0;
   }
+}
+
+// Lattice atoms farther than 0.2a from their site (the test of atom::decide,
+// atom.cpp:33-49) -> far list, before an index built without a decide.
+__global__ __launch_bounds__(kBlock) void kFarScan(Geom G, CXPtr x, const int8_t* __restrict__ type, InterDev in,
+                                                   Ctl* ctl) {
+  int64_t s;
+  int p, cx, cy, cz;
+  if (!siteOfThread(G, s, p, cx, cy, cz)) return;
+  const int8_t t = type[s];
+  if (isVacant(t) || !leftSite(G, p, cx, cy, cz, x.c[0][s], x.c[1][s], x.c[2][s])) return;
+  const unsigned long long q = atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->n_far), 1ull);
+  if (static_cast<int64_t>(q) < in.cap) in.far_new[q] = s;
+  else atomicOr(reinterpret_cast<unsigned long long*>(&ctl->err_flags), 1ull);
 }
 
 // ---------------------------------------------------------------------------
@@ -425,7 +507,7 @@ __global__ __launch_bounds__(kIT) void kInterRho(Geom G, Tables<T> tb, InterView
 template <typename T, int MODE>
 __global__ __launch_bounds__(kIT) void kInterForce(Geom G, Tables<T> tb, InterView iv, CXPtr lx,
                                                    const int8_t* __restrict__ ltype, const T* __restrict__ ldf,
-                                                   InterDev in, VerletParams vp) {
+                                                   int ldfStride, InterDev in, VerletParams vp) {
   const int nl = static_cast<int>(iv.ctl->n_inter);
   const int ne = nl + static_cast<int>(iv.ctl->n_ghost);
   const FullPlaneOffsets& full = *iv.full;
@@ -443,7 +525,7 @@ __global__ __launch_bounds__(kIT) void kInterForce(Geom G, Tables<T> tb, InterVi
         const double d0 = xi0 - lx.c[0][j], d1 = xi1 - lx.c[1][j], d2 = xi2 - lx.c[2][j];
         const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
         if (r2 < G.rc2 && !isVacant(tj)) {
-          const double fp = static_cast<double>(pairForce(tb, ei, species(tj), r2, dfi, ldf[j]));
+          const double fp = static_cast<double>(pairForce(tb, ei, species(tj), r2, dfi, ldf[j * ldfStride]));
           f0 += d0 * fp;
           f1 += d1 * fp;
           f2 += d2 * fp;
@@ -654,7 +736,11 @@ InterView interView(const DevState& s, bool mixed) {
   return v;
 }
 
-void launchDecide(const DevState& s, int mode, hipStream_t st) {
+void launchFarScan(const DevState& s, hipStream_t st) {
+  kFarScan<<<siteGrid(s.G), kBlock, 0, st>>>(s.G, cxp(s.x[s.cur]), s.type, s.in, s.ctl);
+}
+
+void launchDecide(const DevState& s, int mode, bool mixed, hipStream_t st) {
   DecideArgs A;
   A.G = s.G;
   A.mode = mode;
@@ -665,6 +751,7 @@ void launchDecide(const DevState& s, int mode, hipStream_t st) {
   }
   A.ltype = s.type;
   A.lid = s.id;
+  A.lrec = mixed ? s.rec[s.cur] : nullptr;
   A.in = s.in;
   A.ctl = s.ctl;
   kDecide<<<1, kDT, 0, st>>>(A);
@@ -682,16 +769,18 @@ void launchInterRho(const DevState& s, const Tables<double>& t64, const Tables<f
 void launchInterForce(const DevState& s, const Tables<double>& t64, const Tables<float>& t32, bool mixed, int mode,
                       const VerletParams& vp, hipStream_t st) {
   const InterView iv = interView(s, mixed);
-  if (mixed) {
+  const CXPtr lx = cxp(s.x[s.cur]);
+  if (mixed) {  // lattice F' is the w component of the site records
+    const float* ldf = reinterpret_cast<const float*>(s.rec[s.cur]) + 3;
     if (mode == 0)
-      kInterForce<float, 0><<<kIB, kIT, 0, st>>>(s.G, t32, iv, cxp(s.x[s.cur]), s.type, s.df32, s.in, vp);
+      kInterForce<float, 0><<<kIB, kIT, 0, st>>>(s.G, t32, iv, lx, s.type, ldf, 4, s.in, vp);
     else
-      kInterForce<float, 2><<<kIB, kIT, 0, st>>>(s.G, t32, iv, cxp(s.x[s.cur]), s.type, s.df32, s.in, vp);
+      kInterForce<float, 2><<<kIB, kIT, 0, st>>>(s.G, t32, iv, lx, s.type, ldf, 4, s.in, vp);
   } else {
     if (mode == 0)
-      kInterForce<double, 0><<<kIB, kIT, 0, st>>>(s.G, t64, iv, cxp(s.x[s.cur]), s.type, s.df64, s.in, vp);
+      kInterForce<double, 0><<<kIB, kIT, 0, st>>>(s.G, t64, iv, lx, s.type, s.df64, 1, s.in, vp);
     else
-      kInterForce<double, 2><<<kIB, kIT, 0, st>>>(s.G, t64, iv, cxp(s.x[s.cur]), s.type, s.df64, s.in, vp);
+      kInterForce<double, 2><<<kIB, kIT, 0, st>>>(s.G, t64, iv, lx, s.type, s.df64, 1, s.in, vp);
   }
 }
 

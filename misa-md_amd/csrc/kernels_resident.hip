@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
+#include "inter_common.hpp"
 #include "resident.hpp"
 #include "site_common.hpp"
 
@@ -37,7 +38,7 @@ using namespace site;
 // firststep from stored forces + lattice decide + position halo.
 __global__ __launch_bounds__(kBlock) void kFirst(Geom G, CXPtr x, XPtr xn, const int8_t* __restrict__ type,
                                                  XPtr v, CXPtr f, const uint64_t* __restrict__ id, VerletParams vp,
-                                                 StageArgs st, Ctl* ctl) {
+                                                 StageArgs st, Ctl* ctl, float4* __restrict__ recn) {
   int64_t s;
   int p, cx, cy, cz;
   if (!siteOfThread(G, s, p, cx, cy, cz)) return;
@@ -67,6 +68,26 @@ __global__ __launch_bounds__(kBlock) void kFirst(Geom G, CXPtr x, XPtr xn, const
     xn.c[1][q] = y1 + sy * G.L1;
     xn.c[2][q] = y2 + sz * G.L2;
   });
+  if (recn) {
+    const float4 r = isVacant(ti) ? vacantRec() : siteRec(G, p, cx, cy, cz, y0, y1, y2, species(ti));
+    recn[s] = r;
+    forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) { recn[q] = r; });
+  }
+}
+
+// Site records from x and type over every slot, ghosts included.
+__global__ __launch_bounds__(kBlock) void kPackRec(Geom G, CXPtr x, const int8_t* __restrict__ type,
+                                                   float4* __restrict__ rec) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= 2 * G.plane) return;
+  const int p = t >= G.plane ? 1 : 0;
+  int64_t r = t - p * G.plane;
+  const int cx = static_cast<int>(r % G.NX) - G.g;
+  r /= G.NX;
+  const int cy = static_cast<int>(r % G.NY) - G.g;
+  const int cz = static_cast<int>(r / G.NY) - G.g;
+  const int8_t ti = type[t];
+  rec[t] = isVacant(ti) ? vacantRec() : siteRec(G, p, cx, cy, cz, x.c[0][t], x.c[1][t], x.c[2][t], species(ti));
 }
 
 // ---------------------------------------------------------------------------
@@ -127,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void kHaloAll(Geom G, XPtr x, int8_t* type,
 __global__ __launch_bounds__(kBlock) void kDownload(Geom G, AtomElement* __restrict__ aos, int64_t sx, int64_t sy,
                                                     CXPtr x, CXPtr v, CXPtr f, const int8_t* type,
                                                     const uint64_t* id, const double* rho, const double* df64,
-                                                    const float* df32) {
+                                                    const float* dfrec) {  // mixed: w of the site records
   const int64_t r = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (r >= 2 * G.plane) return;
   const int64_t X = r % sx;
@@ -146,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void kDownload(Geom G, AtomElement* __restr
     a.f[k] = inside ? f.c[k][s] : 0.0;
   }
   a.rho = inside ? rho[s] : 0.0;
-  a.df = df64 ? df64[s] : static_cast<double>(df32[s]);
+  a.df = df64 ? df64[s] : static_cast<double>(dfrec[4 * s]);
   aos[r] = a;
 }
 
@@ -209,9 +230,9 @@ __global__ __launch_bounds__(kBlock) void kThermo(Geom G, CXPtr v, CXPtr f, cons
 
 // Potential energy and virial with FP64 tables over the full list:
 // per site F(rho_i) + 1/2 sum_j phi(r_ij) and 1/2 sum_j r_ij^2 fpair_ij.
-__global__ __launch_bounds__(kBlock) void kEnergy(Geom G, LatOffsets O, Tables<double> tb, CXPtr x,
-                                                  const int8_t* __restrict__ type, const double* __restrict__ rho,
-                                                  double* partials) {
+__global__ __launch_bounds__(kBlock) void kEnergy(Geom G, LatOffsets O, const FullPlaneOffsets* __restrict__ full,
+                                                  Tables<double> tb, CXPtr x, const int8_t* __restrict__ type,
+                                                  const double* __restrict__ rho, double* partials) {
   int64_t s;
   int p, cx, cy, cz;
   double vals[3] = {0, 0, 0};
@@ -223,8 +244,10 @@ __global__ __launch_bounds__(kBlock) void kEnergy(Geom G, LatOffsets O, Tables<d
       const double dfi = dEmbed(tb, ei, ri);
       vals[0] = embed(tb, ei, ri);
       const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
-      for (int k = 0; k < O.n[p]; ++k) {
-        const int64_t j = s + O.off[p][k];
+      // the pruned list, and near a far lattice atom the rest of the full list
+      const int nk = O.n[p] + ((ti & kWide) ? full->nw[p] : 0);
+      for (int k = 0; k < nk; ++k) {
+        const int64_t j = s + (k < O.n[p] ? O.off[p][k] : full->woff[p][k - O.n[p]]);
         const int8_t tj = type[j];
         const double d0 = xi0 - x.c[0][j], d1 = xi1 - x.c[1][j], d2 = xi2 - x.c[2][j];
         const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
@@ -265,18 +288,24 @@ void launchUpload(const DevState& s, const AtomElement* aos_dev, int64_t sx, int
 
 void launchHaloAll(const DevState& s, int buf, hipStream_t st) {
   const unsigned grid = static_cast<unsigned>((2 * s.G.plane + kBlock - 1) / kBlock);
-  kHaloAll<<<grid, kBlock, 0, st>>>(s.G, xp(s.x[buf]), s.type, s.df64, s.df32);
+  kHaloAll<<<grid, kBlock, 0, st>>>(s.G, xp(s.x[buf]), s.type, s.df64, nullptr);
 }
 
 void launchDownload(const DevState& s, AtomElement* aos_dev, int64_t sx, int64_t sy, bool mixed, hipStream_t st) {
   const unsigned grid = static_cast<unsigned>((2 * s.G.plane + kBlock - 1) / kBlock);
   kDownload<<<grid, kBlock, 0, st>>>(s.G, aos_dev, sx, sy, cxp(s.x[s.cur]), cxp(s.v), cxp(s.f), s.type, s.id,
-                                     s.rho, mixed ? nullptr : s.df64, s.df32);
+                                     s.rho, mixed ? nullptr : s.df64,
+                                     reinterpret_cast<const float*>(s.rec[s.cur]) + 3);
 }
 
-void launchFirst(const DevState& s, const VerletParams& vp, hipStream_t st) {
+void launchPackRec(const DevState& s, int buf, hipStream_t st) {
+  const unsigned grid = static_cast<unsigned>((2 * s.G.plane + kBlock - 1) / kBlock);
+  kPackRec<<<grid, kBlock, 0, st>>>(s.G, cxp(s.x[buf]), s.type, s.rec[buf]);
+}
+
+void launchFirst(const DevState& s, bool mixed, const VerletParams& vp, hipStream_t st) {
   kFirst<<<siteGrid(s.G), kBlock, 0, st>>>(s.G, cxp(s.x[s.cur]), xp(s.x[1 - s.cur]), s.type, xp(s.v), cxp(s.f),
-                                           s.id, vp, stageOf(s), s.ctl);
+                                           s.id, vp, stageOf(s), s.ctl, mixed ? s.rec[1 - s.cur] : nullptr);
 }
 
 int64_t launchThermo(const DevState& s, hipStream_t st) {
@@ -287,7 +316,7 @@ int64_t launchThermo(const DevState& s, hipStream_t st) {
 
 int64_t launchEnergy(const DevState& s, const LatOffsets& o, const Tables<double>& t64, hipStream_t st) {
   const unsigned grid = siteGrid(s.G);
-  kEnergy<<<grid, kBlock, 0, st>>>(s.G, o, t64, cxp(s.x[s.cur]), s.type, s.rho, s.partials);
+  kEnergy<<<grid, kBlock, 0, st>>>(s.G, o, s.in.full, t64, cxp(s.x[s.cur]), s.type, s.rho, s.partials);
   return grid;
 }
 

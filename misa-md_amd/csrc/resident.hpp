@@ -32,6 +32,10 @@ struct InterDev {
   double* x2[3] = {nullptr, nullptr, nullptr};
   double* v2[3] = {nullptr, nullptr, nullptr};
   int32_t* tmp = nullptr;  // per local atom: recombination target / group
+  // lattice atoms farther than 0.2a from their site (refilled vacancies):
+  // found for the next index, and as flagged by the last one
+  int64_t* far_new = nullptr;
+  int64_t* far_prev = nullptr;
   // new inter atoms found by the lattice Verlet kernels (unordered)
   int64_t* stage_key = nullptr;  // reference linear index of the vacated site
   int64_t* stage_slot = nullptr;
@@ -58,14 +62,20 @@ struct DevState {
   double* f[3] = {nullptr, nullptr, nullptr};
   double* rho = nullptr;
   double* df64 = nullptr;
-  float* df32 = nullptr;
   int8_t* type = nullptr;
   uint64_t* id = nullptr;
+  // mixed precision: site records {u, F'} per slot (ping-pong with x) and the
+  // lattice vector of every pruned offset per centre parity
+  float4* rec[2] = {nullptr, nullptr};
+  float4* dl = nullptr;
   InterDev in;
   Ctl* ctl = nullptr;
   double* partials = nullptr;  // reduction scratch
   int64_t npartials = 0;
 };
+
+// Blocks of the grid-stride inter-atom kernels (and of their partial sums).
+constexpr int kInterBlocks = 64;
 
 struct VerletParams {
   double dt;
@@ -80,7 +90,9 @@ void launchRhoDf(const DevState& s, const LatOffsets& o, const Tables<double>& t
                  bool mixed, int acc, bool store_rho, hipStream_t st);
 void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t64, const Tables<float>& t32,
                  bool mixed, int acc, int mode, const VerletParams& vp, hipStream_t st);
-void launchFirst(const DevState& s, const VerletParams& vp, hipStream_t st);
+void launchFirst(const DevState& s, bool mixed, const VerletParams& vp, hipStream_t st);
+// Site records of buffer `buf` from x and type over every slot (mixed mode).
+void launchPackRec(const DevState& s, int buf, hipStream_t st);
 // Partial sums per block: [mvv, n_lat, n_vac, fx, fy, fz, vmax2] (7 per block)
 int64_t launchThermo(const DevState& s, hipStream_t st);
 // Partial sums per block: [e_embed, e_pair, virial] (3 per block)
@@ -90,8 +102,11 @@ void launchScaleV(const DevState& s, double factor, hipStream_t st);
 // decide mode 0: the step's ownership pass (append new inter atoms, vacancy
 // recombination, migration, ghost images, index, flags); mode 1: migration,
 // ghost images, index and flags only (first force, PKA step, upload).
-void launchDecide(const DevState& s, int mode, hipStream_t st);
+void launchDecide(const DevState& s, int mode, bool mixed, hipStream_t st);
 InterView interView(const DevState& s, bool mixed);
+// Lattice atoms farther than 0.2a from their site -> far list (before a
+// mode-1 decide; in a step only refilled vacancies can be far).
+void launchFarScan(const DevState& s, hipStream_t st);
 void launchInterRho(const DevState& s, const Tables<double>& t64, const Tables<float>& t32, bool mixed,
                     hipStream_t st);
 // mode 0: f stored, v += c f (secondstep); mode 2: f stored only

@@ -93,6 +93,36 @@ __device__ __forceinline__ bool leftSite(const Geom& G, int p, int cx, int cy, i
   return dist > G.thr;
 }
 
+// Site records of the mixed-precision passes: one 16-byte record per slot,
+// {ux, uy, uz, w}.  u is the FP32 displacement of the atom from its lattice
+// site (taken in FP64, then rounded), with the species in the two low
+// mantissa bits of ux (a change of at most 3 ulp); w is F'(rho) of the site,
+// written by the density pass.  A vacancy has a NaN displacement, so every
+// distance test against it fails.  Periodic images carry the same record.
+__device__ __forceinline__ void sitePos(const Geom& G, int p, int cx, int cy, int cz, double& X, double& Y,
+                                        double& Z) {
+  const double h = 0.5 * p;
+  X = (cx + h) * G.a;
+  Y = (cy + h) * G.a;
+  Z = (cz + h) * G.a;
+}
+__device__ __forceinline__ float4 packRec(double u0, double u1, double u2, int sp, float w) {
+  const uint32_t bx = (__float_as_uint(static_cast<float>(u0)) & ~3u) | static_cast<uint32_t>(sp);
+  return make_float4(__uint_as_float(bx), static_cast<float>(u1), static_cast<float>(u2), w);
+}
+__device__ __forceinline__ float4 vacantRec() {
+  const float n = __uint_as_float(0x7fc00000u);
+  return make_float4(n, n, n, 0.0f);
+}
+__device__ __forceinline__ int recSpecies(float ux) { return static_cast<int>(__float_as_uint(ux) & 3u); }
+// Record of the atom at x on site (p, cx, cy, cz) of species sp.
+__device__ __forceinline__ float4 siteRec(const Geom& G, int p, int cx, int cy, int cz, double x0, double x1,
+                                          double x2, int sp) {
+  double X, Y, Z;
+  sitePos(G, p, cx, cy, cz, X, Y, Z);
+  return packRec(x0 - X, x1 - Y, x2 - Z, sp, 0.0f);
+}
+
 // Reference linear index of sub-box cell (cx, cy, cz) of parity p.
 __device__ __forceinline__ int64_t refIndex(const Geom& G, int p, int cx, int cy, int cz) {
   const int64_t sx = 2 * static_cast<int64_t>(G.NX);

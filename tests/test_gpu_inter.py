@@ -204,3 +204,41 @@ def test_inter_rescale(setfl_fecuni):
     _, gi = dev.download()
     np.testing.assert_allclose(gi["v"], sim.inter()["v"], rtol=1e-14)
     assert dev.thermo()["temperature"] == pytest.approx(450.0, rel=1e-12)
+
+
+@pytest.mark.parametrize("mode", [M.PREC_FP64, M.PREC_MIXED])
+def test_far_lattice_atoms_use_the_full_list(setfl_fecuni, mode):
+    """Lattice atoms up to 0.45a from their sites (as a refilled vacancy can
+    be) interact with sites beyond the pruned offset list; the sites around
+    them must visit the whole reference list."""
+    cells = (7, 6, 8)
+    sim = O.Sim(cells, A, F, setfl_fecuni)
+    sim.build_world(77, 300.0, (985, 10, 5), 7)
+    sb = sim.sub_box()
+    rng = np.random.default_rng(77)
+    faces = [(0, 3, 1), (cells[2] - 1, 0, 2 * cells[0] - 1), (4, cells[1] - 1, 0), (cells[2] - 1, cells[1] - 1, 5)]
+    for q in range(12):
+        if q < len(faces):  # next to faces: the partners are periodic images
+            k, j, i = faces[q]
+        else:
+            k, j, i = rng.integers(cells[2]), rng.integers(cells[1]), rng.integers(2 * cells[0])
+        u = np.zeros(3)
+        u[rng.integers(3)] = rng.choice([-1.0, 1.0]) * 0.45 * A
+        sb[k, j, i]["x"] = sb[k, j, i]["x"] + u
+    atoms = sim.atoms().copy()
+    dev = _device(setfl_fecuni, cells)
+    dev.set_precision(mode)
+    dev.upload(atoms)
+    dev.compute_forces()
+    sim.prepare()
+    aos, _ = dev.download()
+    got, want = M.sub_box_view(aos, dev), sim.sub_box()
+    mx, rms = _force_err(got["f"], want["f"])
+    say("far lattice atoms, mode %d: force err max %.3e rms %.3e" % (mode, mx, rms))
+    if mode == M.PREC_FP64:
+        assert mx < 1e-12
+        np.testing.assert_allclose(got["rho"], want["rho"], rtol=1e-12)
+        t, e = dev.thermo(), sim.energy()
+        assert abs(t["e_potential"] - e["e_potential"]) / (2 * np.prod(cells)) < 1e-13
+    else:
+        assert rms < 1e-5
