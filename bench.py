@@ -1,15 +1,20 @@
-"""EAM step throughput on one MI355X: atom-steps/s of the 16M-atom FeCuNi box.
+"""EAM step throughput on one MI355X: atom-steps/s of the 16M-atom FeCuNi cascade.
 
 Workload: BCC FeCuNi, 200^3 cells (16,000,000 atoms), Fe:Cu:Ni = 985:10:5
-placed by the reference's world builder, 300 K, dt = 1 fs, synthetic
-3-element setfl table (Nr = Nrho = 5000, cutoff 5.6 A).  A step is one
-complete velocity-Verlet step of the resident state (rho/F' pass, force +
-Verlet pass, ownership decisions, periodic halo), state resident in HBM.
+placed by the reference's world builder at 300 K, synthetic 3-element setfl
+table (Nr = Nrho = 5000, cutoff 5.6 A).  The box is equilibrated (untimed:
+steps of 1 fs with a rescale to 300 K), then a primary knock-on atom at the
+box centre gets 5 keV along [1 3 5] and the cascade runs with an adaptive step
+dt = min(1 fs, 0.01 A / max|v|).  A step is one complete velocity-Verlet step
+of the resident state (rho/F' pass, force pass, step length, half kicks and
+drift, ownership decisions with inter atoms, periodic halo), state resident
+in HBM; the timed steps are the first steps of the cascade.
 
-Prints one JSON line: the FP32-mixed rate is `value`; the plain FP64 rate,
-the mixed-vs-FP64 force error, the roofline of the dominant kernel (from HIP
-events on the library's stream) and the single-core CPU oracle rate on a
-bounded sample of the same workload sit beside it.
+Prints one JSON line: the FP32-mixed rate is `value`; the plain FP64 rate
+from the same equilibrated state, the mixed-vs-FP64 force error, the
+roofline of the dominant kernel (HIP events on the library's stream over the
+timed steps) and the single-core CPU oracle rate on a bounded sample sit
+beside it.
 """
 import argparse
 import json
@@ -43,11 +48,31 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_precision(dev, mode, steps, warmup):
-    """Warm-up steps, then `steps` timed steps; HIP events around every
-    launch on the library's stream split the timed region by kernel."""
-    dev.set_precision(mode)
+PKA_DIR = (1.0, 3.0, 5.0)  # the recoil direction of the cascade configuration
+
+
+def equilibrate(dev, steps):
+    """Untimed: fixed 1 fs steps with a rescale to 300 K every 50 steps."""
+    dev.set_precision(M.PREC_MIXED)
+    dev.set_adaptive_dt(0.001, 0.0)
     dev.compute_forces()
+    for _ in range(max(1, steps // 50)):
+        dev.step(50)
+        dev.rescale(300.0)
+    return dev.download()
+
+
+def run_precision(dev, state, mode, steps, warmup, pka_energy, pka_site):
+    """From the equilibrated state: the recoil, warm-up steps, then `steps`
+    timed steps; HIP events around every launch on the library's stream split
+    the timed region by kernel."""
+    dev.upload(*state)
+    dev.set_precision(mode)
+    dev.set_adaptive_dt(0.001, 0.0)
+    dev.compute_forces()
+    if pka_energy > 0:
+        dev.collision_step(pka_site, PKA_DIR, pka_energy)
+        dev.set_adaptive_dt(0.001, 0.01)
     if warmup:
         dev.step(warmup)
     dev.set_profiling(True)
@@ -56,7 +81,10 @@ def run_precision(dev, mode, steps, warmup):
     secs = time.perf_counter() - t0
     tim = dev.last_timings()
     dev.set_profiling(False)
-    return secs, tim
+    t_sim, dt_last = dev.sim_time()
+    th = dev.thermo()
+    return secs, tim, {"t_sim_ps": t_sim, "dt_last_ps": dt_last, "n_inter": th["n_inter"],
+                       "n_vacancy": th["n_vacancy"], "max_speed": th["max_speed"]}
 
 
 def traffic_bytes():
@@ -112,6 +140,8 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cells", type=int, default=200)
+    ap.add_argument("--equil", type=int, default=100, help="untimed equilibration steps")
+    ap.add_argument("--pka-energy", type=float, default=5000.0, help="eV; 0 times equilibrium steps of 1 fs")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -129,10 +159,14 @@ def main():
         dev.set_potential(pot)
         dev.build_world(2020, 300.0, (985, 10, 5), 1)
         dev.set_timestep(0.001)
+        log("equilibration")
+        state = equilibrate(dev, args.equil)
+        site = tuple(n // 2 for _ in range(3)) + (0,)
         log("fp64 run")
-        s64, t64 = run_precision(dev, M.PREC_FP64, args.steps, args.warmup)
+        s64, t64, c64 = run_precision(dev, state, M.PREC_FP64, args.steps, args.warmup, args.pka_energy, site)
         log("mixed run")
-        s32, t32 = run_precision(dev, M.PREC_MIXED, args.steps, args.warmup)
+        s32, t32, c32 = run_precision(dev, state, M.PREC_MIXED, args.steps, args.warmup, args.pka_energy, site)
+        del state
         log("force error")
         fmax, frms, f_scale = force_error(dev)
         th = dev.thermo()
@@ -171,9 +205,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 (mixed: f32 splines/pair terms, f64 positions/Verlet/sums)",
         "data": "synthetic",
-        "config": {"workload": "BCC FeCuNi %d^3 cells (%d atoms), Fe:Cu:Ni 985:10:5, 300 K, dt 1 fs, "
-                               "synthetic 3-element setfl Nr=Nrho=5000 rc 5.6 A" % (n, natoms),
+        "config": {"workload": ("collision cascade: BCC FeCuNi %d^3 cells (%d atoms), Fe:Cu:Ni 985:10:5, "
+                                "equilibrated at 300 K (%d steps of 1 fs), PKA %.0f eV at %s along [1 3 5], "
+                                "adaptive dt = min(1 fs, 0.01 A / max|v|), timed steps %d after %d warm-up; "
+                                "synthetic 3-element setfl Nr=Nrho=5000 rc 5.6 A")
+                               % (n, natoms, args.equil, args.pka_energy, str(site), args.steps, args.warmup)
+                               if args.pka_energy > 0 else
+                               "BCC FeCuNi %d^3 cells (%d atoms), Fe:Cu:Ni 985:10:5, 300 K, dt 1 fs, synthetic "
+                               "3-element setfl Nr=Nrho=5000 rc 5.6 A" % (n, natoms),
                    "atoms": natoms, "parallelism": "single GPU"},
+        "cascade": {"mixed": c32, "fp64": c64},
         "fp64": {"value": rate64, "unit": "atom-steps/s", "ms_per_step": s64 * 1e3 / args.steps},
         "kernels": kernels,
         "mixed_over_fp64": rate32 / rate64,

@@ -37,6 +37,7 @@ extern "C" {
 #endif
 
 typedef struct misa_handle misa_handle;
+typedef struct misa_batch misa_batch;
 
 /* error codes */
 #define MISA_OK 0
@@ -139,6 +140,14 @@ int misa_build_world(misa_handle* h, uint32_t seed, double tset, const int32_t r
 int misa_set_precision(misa_handle* h, int mode);
 /* NewtonMotion::setTimestepLength (newton_motion.cpp:13-17). */
 int misa_set_timestep(misa_handle* h, double dt);
+/* Adaptive step length (no counterpart in the reference, which only switches
+ * dt per stage, md_simulation.cpp:42-58): with max_disp > 0 every step of
+ * misa_step takes dt = min(dt_max, max_disp / max_i |v_i|) over lattice and
+ * inter atoms, computed on the device from the velocities at the start of the
+ * step; max_disp = 0 returns to the fixed step of misa_set_timestep. */
+int misa_set_adaptive_dt(misa_handle* h, double dt_max, double max_disp);
+/* Simulated time (ps) since the last upload, and the length of the last step. */
+int misa_sim_time(misa_handle* h, double* t_ps, double* dt_last);
 /* Inter-atom list capacity (default 65536 entries). */
 int misa_set_inter_capacity(misa_handle* h, int64_t cap);
 
@@ -168,6 +177,26 @@ int misa_last_timings(const misa_handle* h, double out[4]);
  * misa_last_launch_counts gives the number of launches per class. */
 int misa_set_profiling(misa_handle* h, int on);
 int misa_last_launch_counts(const misa_handle* h, int64_t out[3]);
+
+/* ---- batches of independent boxes (no counterpart in the reference, which
+ *      runs one box per MPI job) ----
+ * A batch steps many configured handles of one device together (replicas of
+ * other sizes, temperatures, compositions or cascades).  Each step's launches
+ * of every box are captured once into a hipGraph (two loop bodies per graph,
+ * every box an independent branch) and replayed, so the boxes' kernels
+ * overlap on the GPU and cost no per-launch host work.  The graph is
+ * re-captured when a box changes (precision, step, offsets, potential,
+ * capacity, tile shape).  The handles stay owned by the caller. */
+int misa_batch_create(misa_handle* const* handles, int n, misa_batch** out);
+int misa_batch_destroy(misa_batch* b);
+const char* misa_batch_last_error(const misa_batch* b);
+/* `nsteps` loop bodies of every box, as misa_step would run them one box at
+ * a time; `graph_replays` (may be NULL) returns the replays so far. */
+int misa_batch_step(misa_batch* b, int64_t nsteps, int64_t* graph_replays);
+/* Autotuner of the site kernels' tile shape: times `steps` steps for every
+ * y-tile height dividing the box (up to 32) and keeps the fastest; the trial
+ * steps advance the simulation. */
+int misa_autotune(misa_handle* h, int64_t steps, int32_t* best_ty, double* best_ms);
 
 /* ---- host-only set-up: no device access, usable without a GPU ---- */
 

@@ -41,8 +41,12 @@ struct misa_handle {
   int precision = MISA_PREC_FP64;
   int acc = 0;
   double dt = 0.001;
+  // adaptive step (max_disp > 0): dt = min(dt_max, max_disp / max|v|) per step
+  double dt_max = 0.001, max_disp = 0.0;
+  double t_fixed = 0.0;  // simulated time of the fixed-step calls (ps)
   bool state_ready = false;
   bool forces_ready = false;
+  uint64_t gen = 0;  // bumped by every change a captured step graph would bake in
   AtomElement* aos_dev = nullptr;
   std::vector<void*> allocs;
   bool profile = false;
@@ -142,6 +146,7 @@ int buildPlaneOffsets(misa_handle* h) {
   for (int p = 0; p < 2; p++)
     for (int k = 0; k < h->lat.n[p]; k++) h->lat.sh[p][k] = static_cast<uint32_t>(h->lat.off[p][k] - lo);
   h->have_offsets = true;
+  h->gen++;
   return MISA_OK;
 }
 
@@ -238,6 +243,7 @@ int allocInter(misa_handle* h, int64_t cap) {
   in.cap = cap;
   in.ecap = ecap;
   in.kcap = kcap;
+  h->gen++;
   if (hipMemset(base, 0, bytes) != hipSuccess || hipMemset(in.head, 0xff, slots * 4) != hipSuccess ||
       hipMemset(in.site_claim, 0x7f, slots * 4) != hipSuccess ||
       hipMemcpy(in.full, &h->fullPlane, sizeof(FullPlaneOffsets), hipMemcpyHostToDevice) != hipSuccess)
@@ -422,6 +428,7 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
   std::string e;
   Setfl& sf = h->setfl;
   if (!parseSetfl(path, sf, e)) return fail(h, MISA_E_IO, e);
+  h->gen++;
   h->cF.assign(sf.nel, {});
   h->cR.assign(sf.nel, {});
   h->cZ.assign(sf.z2.size(), {});
@@ -585,6 +592,7 @@ int misa_upload_state(misa_handle* h, const void* atoms, const void* inter, int6
   launchHaloAll(h->s, h->s.cur, h->stream);
   HIPCHECK(h, hipGetLastError());
   HIPCHECK(h, hipMemsetAsync(h->s.ctl, 0, sizeof(Ctl), h->stream));
+  h->t_fixed = 0.0;
   HIPCHECK(h, hipMemsetAsync(h->s.in.head, 0xff, static_cast<size_t>(2 * h->s.G.plane) * 4, h->stream));
   if (n_inter > 0) {
     if (h->interAosCap < n_inter) {
@@ -645,6 +653,7 @@ int misa_build_world(misa_handle* h, uint32_t seed, double tset, const int32_t r
 
 int misa_set_precision(misa_handle* h, int mode) {
   if (!h) return MISA_E_ARG;
+  h->gen++;
   if (mode == MISA_PREC_FP64) {
     h->precision = mode;
     h->acc = 0;
@@ -663,7 +672,26 @@ int misa_set_precision(misa_handle* h, int mode) {
 
 int misa_set_timestep(misa_handle* h, double dt) {
   if (!h || !(dt > 0)) return MISA_E_ARG;
+  h->gen++;
   h->dt = dt;
+  return MISA_OK;
+}
+
+int misa_set_adaptive_dt(misa_handle* h, double dt_max, double max_disp) {
+  if (!h || !(dt_max > 0) || max_disp < 0) return MISA_E_ARG;
+  h->gen++;
+  h->dt_max = dt_max;
+  h->max_disp = max_disp;
+  return MISA_OK;
+}
+
+int misa_sim_time(misa_handle* h, double* t_ps, double* dt_last) {
+  if (!h || !t_ps) return MISA_E_ARG;
+  HIPCHECK(h, hipSetDevice(h->device));
+  Ctl c;
+  HIPCHECK(h, hipMemcpy(&c, h->s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+  *t_ps = h->t_fixed + c.t_sim;
+  if (dt_last) *dt_last = h->max_disp > 0.0 ? c.dt : h->dt;
   return MISA_OK;
 }
 
@@ -717,54 +745,60 @@ int misa_compute_forces(misa_handle* h) {
   return MISA_OK;
 }
 
-int misa_step(misa_handle* h, int64_t nsteps) {
-  if (!h || nsteps < 0) return MISA_E_ARG;
-  if (!h->forces_ready) return fail(h, MISA_E_ARG, "misa_compute_forces must run before misa_step");
-  if (nsteps == 0) return MISA_OK;
-  HIPCHECK(h, hipSetDevice(h->device));
-  const VerletParams vp = verletOf(h->dt);
-  const bool mixed = h->precision == MISA_PREC_MIXED;
-  for (auto e : h->events) hipEventDestroy(e);
-  h->events.clear();
-  h->event_cls.clear();
-  hipEvent_t e0, e1;
-  HIPCHECK(h, hipEventCreate(&e0));
-  HIPCHECK(h, hipEventCreate(&e1));
-  HIPCHECK(h, hipEventRecord(e0, h->stream));
-  // firststep + decide of the first step (simulation.cpp:168-176)
+}  // extern "C"
+
+namespace {
+
+// The loop body of simulation::simulate (simulation.cpp:164-194) as launch
+// sequences on the handle's stream, without synchronisation:
+//   prologue: (adaptive: step length) firststep of step 0, decide
+//   body(i):  forces of step i and its secondstep; unless last, the
+//             firststep of step i+1 (fused into the force pass with a fixed
+//             step) and its decide
+struct StepPlan {
+  bool adaptive, mixed;
+  VerletParams vp;
+};
+
+StepPlan planOf(misa_handle* h) {
+  StepPlan p;
+  p.adaptive = h->max_disp > 0.0;
+  p.mixed = h->precision == MISA_PREC_MIXED;
+  p.vp = verletOf(h->dt);
+  if (p.adaptive) p.vp.dtp = &h->s.ctl->dt;
+  return p;
+}
+
+void enqFirstAndDecide(misa_handle* h, const StepPlan& p, bool lattice) {
   {
     Timer t(h, 2);
-    launchFirst(h->s, mixed, vp, h->stream);
-    launchInterFirst(h->s, vp, h->stream);
+    if (p.adaptive) launchAdaptiveDt(h->s, h->dt_max, h->max_disp, h->stream);
+    if (lattice) launchFirst(h->s, p.mixed, p.vp, h->stream);
+    launchInterFirst(h->s, p.vp, h->stream);
   }
   h->s.cur = 1 - h->s.cur;
   {
     Timer t(h, 2);
-    launchDecide(h->s, 0, mixed, h->stream);
+    launchDecide(h->s, 0, p.mixed, h->stream);
   }
-  for (int64_t i = 0; i < nsteps; i++) {
-    const bool last = i == nsteps - 1;
-    // forces of step i, its secondstep, and (fused) the firststep of step i+1
-    forcePass(h, last ? 0 : 1, 0, last, vp);
-    if (!last) {
-      {
-        Timer t(h, 2);
-        launchInterFirst(h->s, vp, h->stream);
-      }
-      h->s.cur = 1 - h->s.cur;
-      {
-        Timer t(h, 2);
-        launchDecide(h->s, 0, mixed, h->stream);
-      }
-    }
-  }
-  HIPCHECK(h, hipGetLastError());
-  HIPCHECK(h, hipEventRecord(e1, h->stream));
+}
+
+void enqBody(misa_handle* h, const StepPlan& p, bool last) {
+  const bool fuse = !last && !p.adaptive;
+  forcePass(h, fuse ? 1 : 0, 0, last, p.vp);
+  if (!last) enqFirstAndDecide(h, p, !fuse);
+}
+
+// Error flags of the step kernels; timings of the profiled launches.
+int finishSteps(misa_handle* h, int64_t nsteps, hipEvent_t e0, hipEvent_t e1) {
+  if (h->max_disp <= 0.0) h->t_fixed += static_cast<double>(nsteps) * h->dt;
   const int rc = syncAndCheck(h);
   float ms = 0;
-  hipEventElapsedTime(&ms, e0, e1);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  if (e0 && e1) {
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+  }
   h->timings[0] = h->timings[1] = h->timings[2] = 0;
   h->launches[0] = h->launches[1] = h->launches[2] = 0;
   for (size_t k = 0; k < h->event_cls.size(); k++) {
@@ -774,11 +808,36 @@ int misa_step(misa_handle* h, int64_t nsteps) {
     h->launches[h->event_cls[k]]++;
   }
   h->timings[3] = ms;
-  if (rc != MISA_OK) {
-    h->forces_ready = false;
-    return rc;
-  }
-  return MISA_OK;
+  if (rc != MISA_OK) h->forces_ready = false;
+  return rc;
+}
+
+void clearEvents(misa_handle* h) {
+  for (auto e : h->events) hipEventDestroy(e);
+  h->events.clear();
+  h->event_cls.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
+int misa_step(misa_handle* h, int64_t nsteps) {
+  if (!h || nsteps < 0) return MISA_E_ARG;
+  if (!h->forces_ready) return fail(h, MISA_E_ARG, "misa_compute_forces must run before misa_step");
+  if (nsteps == 0) return MISA_OK;
+  HIPCHECK(h, hipSetDevice(h->device));
+  const StepPlan p = planOf(h);
+  clearEvents(h);
+  hipEvent_t e0, e1;
+  HIPCHECK(h, hipEventCreate(&e0));
+  HIPCHECK(h, hipEventCreate(&e1));
+  HIPCHECK(h, hipEventRecord(e0, h->stream));
+  enqFirstAndDecide(h, p, true);
+  for (int64_t i = 0; i < nsteps; i++) enqBody(h, p, i == nsteps - 1);
+  HIPCHECK(h, hipGetLastError());
+  HIPCHECK(h, hipEventRecord(e1, h->stream));
+  return finishSteps(h, nsteps, e0, e1);
 }
 
 int misa_setv(misa_handle* h, const int32_t lat[4], const double dir[3], double energy) {
@@ -910,6 +969,214 @@ int misa_last_launch_counts(const misa_handle* h, int64_t out[3]) {
 }
 
 // ---- host-only set-up (no device access) ----
+
+// ---- batches of independent boxes: captured step graphs ----
+
+}  // extern "C"
+
+struct misa_batch {
+  std::vector<misa_handle*> hs;
+  int device = 0;
+  std::string err;
+  hipStream_t origin = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> done;  // per box: its stream reached the graph
+  hipEvent_t after = nullptr;    // the origin stream finished the replays
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint64_t> key;  // per box: generation and position buffer the graph was captured for
+  int64_t replays = 0;
+};
+
+namespace {
+
+int bfail(misa_batch* b, int code, const std::string& msg) {
+  b->err = msg;
+  return code;
+}
+
+#define BHIP(b, call)                                                                               \
+  do {                                                                                              \
+    hipError_t e_ = (call);                                                                         \
+    if (e_ != hipSuccess) return bfail(b, MISA_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+std::vector<uint64_t> batchKey(const misa_batch* b) {
+  std::vector<uint64_t> k;
+  for (const misa_handle* h : b->hs) {
+    k.push_back(h->gen);
+    k.push_back(static_cast<uint64_t>(h->s.cur));
+  }
+  return k;
+}
+
+// Capture two loop bodies of every box, each box a branch of its own stream
+// forked from and joined back into the origin stream.
+int captureBatch(misa_batch* b, const std::vector<StepPlan>& plans) {
+  if (b->exec) {
+    hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  BHIP(b, hipStreamBeginCapture(b->origin, hipStreamCaptureModeRelaxed));
+  hipError_t e = hipEventRecord(b->fork, b->origin);
+  for (size_t i = 0; i < b->hs.size() && e == hipSuccess; i++) {
+    misa_handle* h = b->hs[i];
+    e = hipStreamWaitEvent(h->stream, b->fork, 0);
+    if (e != hipSuccess) break;
+    enqBody(h, plans[i], false);
+    enqBody(h, plans[i], false);
+    e = hipEventRecord(b->done[i], h->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->origin, b->done[i], 0);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(b->origin, &g);
+  if (e != hipSuccess) return bfail(b, MISA_E_HIP, std::string("capturing the step graph: ") + hipGetErrorString(e));
+  if (e2 != hipSuccess) return bfail(b, MISA_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+  const hipError_t e3 = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e3 != hipSuccess) return bfail(b, MISA_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
+  b->key = batchKey(b);
+  return MISA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int misa_batch_create(misa_handle* const* handles, int n, misa_batch** out) {
+  if (!handles || n < 1 || !out) return MISA_E_ARG;
+  *out = nullptr;
+  misa_batch* b = new misa_batch();
+  b->device = handles[0]->device;
+  for (int i = 0; i < n; i++) {
+    if (!handles[i] || handles[i]->device != b->device) {
+      delete b;
+      return MISA_E_ARG;
+    }
+    for (int j = 0; j < i; j++)
+      if (handles[j] == handles[i]) {
+        delete b;
+        return MISA_E_ARG;
+      }
+    b->hs.push_back(handles[i]);
+  }
+  b->done.assign(n, nullptr);
+  hipError_t e = hipSetDevice(b->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->after, hipEventDisableTiming);
+  for (int i = 0; i < n && e == hipSuccess; i++) e = hipEventCreateWithFlags(&b->done[i], hipEventDisableTiming);
+  if (e != hipSuccess) {
+    misa_batch_destroy(b);
+    return MISA_E_HIP;
+  }
+  *out = b;
+  return MISA_OK;
+}
+
+int misa_batch_destroy(misa_batch* b) {
+  if (!b) return MISA_E_ARG;
+  hipSetDevice(b->device);
+  if (b->origin) hipStreamSynchronize(b->origin);
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  for (hipEvent_t e : b->done)
+    if (e) hipEventDestroy(e);
+  if (b->fork) hipEventDestroy(b->fork);
+  if (b->after) hipEventDestroy(b->after);
+  if (b->origin) hipStreamDestroy(b->origin);
+  delete b;
+  return MISA_OK;
+}
+
+const char* misa_batch_last_error(const misa_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int misa_batch_step(misa_batch* b, int64_t nsteps, int64_t* graph_replays) {
+  if (!b || nsteps < 0) return MISA_E_ARG;
+  for (misa_handle* h : b->hs)
+    if (!h->forces_ready) return bfail(b, MISA_E_ARG, "misa_compute_forces must run on every box before stepping");
+  if (nsteps == 0) return MISA_OK;
+  BHIP(b, hipSetDevice(b->device));
+  const size_t n = b->hs.size();
+  std::vector<StepPlan> plans;
+  std::vector<bool> prof(n);
+  for (size_t i = 0; i < n; i++) {
+    misa_handle* h = b->hs[i];
+    prof[i] = h->profile;
+    h->profile = false;  // no per-launch events inside graphs
+    clearEvents(h);
+    plans.push_back(planOf(h));
+  }
+  // prologue of every box on its own stream
+  for (size_t i = 0; i < n; i++) enqFirstAndDecide(b->hs[i], plans[i], true);
+  const int64_t pairs = (nsteps - 1) / 2;
+  int rc = MISA_OK;
+  if (pairs > 0) {
+    if (!b->exec || b->key != batchKey(b)) rc = captureBatch(b, plans);
+    if (rc == MISA_OK) {
+      for (size_t i = 0; i < n && rc == MISA_OK; i++) {
+        if (hipEventRecord(b->done[i], b->hs[i]->stream) != hipSuccess ||
+            hipStreamWaitEvent(b->origin, b->done[i], 0) != hipSuccess)
+          rc = bfail(b, MISA_E_HIP, "ordering the graph after the prologue failed");
+      }
+      for (int64_t r = 0; r < pairs && rc == MISA_OK; r++)
+        if (hipGraphLaunch(b->exec, b->origin) != hipSuccess) rc = bfail(b, MISA_E_HIP, "hipGraphLaunch failed");
+      if (rc == MISA_OK && hipEventRecord(b->after, b->origin) != hipSuccess)
+        rc = bfail(b, MISA_E_HIP, "hipEventRecord failed");
+      for (size_t i = 0; i < n && rc == MISA_OK; i++)
+        if (hipStreamWaitEvent(b->hs[i]->stream, b->after, 0) != hipSuccess)
+          rc = bfail(b, MISA_E_HIP, "ordering the epilogue after the graph failed");
+      b->replays += pairs;
+    }
+  }
+  // the steps the graph replays did not cover, then the last body
+  const int64_t eager = (nsteps - 1) - 2 * (rc == MISA_OK ? pairs : 0);
+  for (size_t i = 0; i < n; i++) {
+    misa_handle* h = b->hs[i];
+    for (int64_t k = 0; k < eager; k++) enqBody(h, plans[i], false);
+    enqBody(h, plans[i], true);
+  }
+  for (size_t i = 0; i < n; i++) {
+    misa_handle* h = b->hs[i];
+    const int r = finishSteps(h, nsteps, nullptr, nullptr);
+    h->profile = prof[i];
+    if (r != MISA_OK && rc == MISA_OK) rc = bfail(b, r, "box " + std::to_string(i) + ": " + h->err);
+  }
+  if (graph_replays) *graph_replays = b->replays;
+  return rc;
+}
+
+// Tile shape of the site kernels: try every y-tile height that divides the
+// box (up to 32 rows) over `steps` steps each and keep the fastest.  The trial
+// steps are real steps of the simulation.
+int misa_autotune(misa_handle* h, int64_t steps, int32_t* best_ty, double* best_ms) {
+  if (!h || steps < 1) return MISA_E_ARG;
+  if (!h->forces_ready) return fail(h, MISA_E_ARG, "misa_compute_forces must run before misa_autotune");
+  HIPCHECK(h, hipSetDevice(h->device));
+  const bool prof = h->profile;
+  h->profile = false;
+  int bestTy = h->s.G.W.ty;
+  double bestMs = 1e300;
+  for (int ty = 1; ty <= std::min(32, h->s.G.n1); ty++) {
+    if (h->s.G.n1 % ty) continue;
+    h->s.G.W.ty = ty;
+    h->gen++;
+    const int rc = misa_step(h, steps);
+    if (rc != MISA_OK) {
+      h->profile = prof;
+      return rc;
+    }
+    const double ms = h->timings[3] / static_cast<double>(steps);
+    if (ms < bestMs) {
+      bestMs = ms;
+      bestTy = ty;
+    }
+  }
+  h->s.G.W.ty = bestTy;
+  h->gen++;
+  h->profile = prof;
+  if (best_ty) *best_ty = bestTy;
+  if (best_ms) *best_ms = bestMs;
+  return MISA_OK;
+}
 
 int misa_host_offsets(const misa_geom* g, int which, int64_t* out, int64_t cap, int64_t* n) {
   if (!g || !n || which < 0 || which > 5) return MISA_E_ARG;

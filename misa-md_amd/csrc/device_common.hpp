@@ -69,8 +69,17 @@ struct Ctl {
   int64_t err_flags;    // bit 0: inter capacity, bit 1: ghost capacity, bit 2: non-finite, bit 3: left the box
   int64_t n_far;        // lattice atoms farther than 0.2a from their site, found for the next index
   int64_t n_far_prev;   // the same, as flagged by the last decide
-  int64_t pad;
+  double dt;            // adaptive step: length of the current step (ps)
+  double t_sim;         // adaptive step: simulated time accumulated on the device (ps)
+  unsigned long long vmax2;  // adaptive step: max |v|^2 (bits of a non-negative double)
+  int64_t pad[5];
 };
+
+// Masses and the force-to-velocity unit factor (atom_types.h:17-19,
+// pre_define.h:11-21) for the kernels that derive the half-kick factors from
+// a step length held in device memory.
+__device__ __forceinline__ double speciesMass(int t) { return t == 0 ? 55.845 : (t == 1 ? 63.546 : 58.6934); }
+constexpr double kFtm2vDev = 1.0 / 1.0364269e-4;
 
 // Pruned lattice-lattice neighbour list: slot deltas per centre parity, and
 // the same deltas shifted by `lo` (the most negative delta) so that a buffer

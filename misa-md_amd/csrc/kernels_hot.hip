@@ -338,7 +338,8 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
   }
   double f0, f1, f2;
   sum3Value<ACC>(acc, f0, f1, f2);
-  const double c = vp.c[ei];
+  const StepDt sd = stepDt(vp);
+  const double c = sd.c[ei];
   double v0 = v.c[0][s] + c * f0;
   double v1 = v.c[1][s] + c * f1;
   double v2 = v.c[2][s] + c * f2;
@@ -360,9 +361,9 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
   v0 = v0 + c * f0;
   v1 = v1 + c * f1;
   v2 = v2 + c * f2;
-  const double y0 = xi0 + vp.dt * v0;
-  const double y1 = xi1 + vp.dt * v1;
-  const double y2 = xi2 + vp.dt * v2;
+  const double y0 = xi0 + sd.dt * v0;
+  const double y1 = xi1 + sd.dt * v1;
+  const double y2 = xi2 + sd.dt * v2;
   if (leftSite(G, p, cx, cy, cz, y0, y1, y2)) {
     stageInter(st, ctl, refIndex(G, p, cx, cy, cz), s, id[s], ti, y0, y1, y2, v0, v1, v2);
     v0 = v1 = v2 = 0.0;
@@ -601,7 +602,8 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
   }
   double f0, f1, f2;
   sum3Value<ACC>(acc, f0, f1, f2);
-  const double c = vp.c[ei];
+  const StepDt sd = stepDt(vp);
+  const double c = sd.c[ei];
   double v0 = v.c[0][s] + c * f0;
   double v1 = v.c[1][s] + c * f1;
   double v2 = v.c[2][s] + c * f2;
@@ -619,9 +621,9 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
   v0 = v0 + c * f0;
   v1 = v1 + c * f1;
   v2 = v2 + c * f2;
-  const double y0 = xi0 + vp.dt * v0;
-  const double y1 = xi1 + vp.dt * v1;
-  const double y2 = xi2 + vp.dt * v2;
+  const double y0 = xi0 + sd.dt * v0;
+  const double y1 = xi1 + sd.dt * v1;
+  const double y2 = xi2 + sd.dt * v2;
   if (leftSite(G, p, cx, cy, cz, y0, y1, y2)) {
     stageInter(st, ctl, refIndex(G, p, cx, cy, cz), s, id[s], ti, y0, y1, y2, v0, v1, v2);
     v0 = v1 = v2 = 0.0;

@@ -553,7 +553,7 @@ __global__ __launch_bounds__(kIT) void kInterForce(Geom G, Tables<T> tb, InterVi
     in.f[1][i] = f1;
     in.f[2][i] = f2;
     if (MODE == 0) {
-      const double c = vp.c[ei];
+      const double c = stepDt(vp).c[ei];
       in.v[0][i] += c * f0;
       in.v[1][i] += c * f1;
       in.v[2][i] += c * f2;
@@ -565,11 +565,12 @@ __global__ __launch_bounds__(kIT) void kInterForce(Geom G, Tables<T> tb, InterVi
 __global__ __launch_bounds__(kIT) void kInterFirst(InterDev in, const Ctl* ctl, VerletParams vp) {
   const int nl = static_cast<int>(ctl->n_inter);
   for (int i = blockIdx.x * kIT + threadIdx.x; i < nl; i += kIT * kIB) {
-    const double c = vp.c[species(in.type[i])];
+    const StepDt sd = stepDt(vp);
+    const double c = sd.c[species(in.type[i])];
     for (int d = 0; d < 3; d++) {
       const double v = in.v[d][i] + c * in.f[d][i];
       in.v[d][i] = v;
-      in.x[d][i] += vp.dt * v;
+      in.x[d][i] += sd.dt * v;
     }
   }
 }
@@ -718,7 +719,67 @@ __global__ __launch_bounds__(kIT) void kInterEnergy(Geom G, Tables<double> tb, I
   blockSumI<3>(vals, partials + 3 * blockIdx.x);
 }
 
+// ---------------------------------------------------------------------------
+// adaptive step length: dt = min(dt_max, max_disp / max|v|)
+// ---------------------------------------------------------------------------
+// |v|^2 with each product and sum rounded separately; non-negative doubles
+// order like their bit patterns, so an integer atomic max finds the largest.
+__device__ __forceinline__ unsigned long long speed2Bits(double a, double b, double c) {
+#pragma clang fp contract(off)
+  const double v2 = (a * a + b * b) + c * c;
+  return static_cast<unsigned long long>(__double_as_longlong(v2));
+}
+
+template <int N>
+__device__ __forceinline__ void blockMaxToCtl(unsigned long long m, Ctl* ctl) {
+  __shared__ unsigned long long red[N / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long y = __shfl_down(m, o, 64);
+    m = y > m ? y : m;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long b = 0;
+    for (int k = 0; k < N / 64; k++) b = red[k] > b ? red[k] : b;
+    atomicMax(&ctl->vmax2, b);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void kVmaxSites(Geom G, CXPtr v, const int8_t* __restrict__ type, Ctl* ctl) {
+  int64_t s;
+  int p, cx, cy, cz;
+  unsigned long long m = 0;
+  if (siteOfThread(G, s, p, cx, cy, cz) && !isVacant(type[s])) m = speed2Bits(v.c[0][s], v.c[1][s], v.c[2][s]);
+  blockMaxToCtl<kBlock>(m, ctl);
+}
+
+__global__ __launch_bounds__(kIT) void kVmaxInter(InterDev in, Ctl* ctl) {
+  const int nl = static_cast<int>(ctl->n_inter);
+  unsigned long long m = 0;
+  for (int i = blockIdx.x * kIT + threadIdx.x; i < nl; i += kIT * kIB) {
+    const unsigned long long b = speed2Bits(in.v[0][i], in.v[1][i], in.v[2][i]);
+    m = b > m ? b : m;
+  }
+  blockMaxToCtl<kIT>(m, ctl);
+}
+
+__global__ void kDtUpdate(Ctl* ctl, double dt_max, double max_disp) {
+  const double vmax = sqrt(__longlong_as_double(static_cast<long long>(ctl->vmax2)));
+  const double dt = vmax > 0.0 ? fmin(dt_max, max_disp / vmax) : dt_max;
+  ctl->dt = dt;
+  ctl->t_sim += dt;
+  ctl->vmax2 = 0ull;
+}
+
 }  // namespace
+
+void launchAdaptiveDt(const DevState& s, double dt_max, double max_disp, hipStream_t st) {
+  kVmaxSites<<<siteGrid(s.G), kBlock, 0, st>>>(s.G, cxp(s.v), s.type, s.ctl);
+  kVmaxInter<<<kIB, kIT, 0, st>>>(s.in, s.ctl);
+  kDtUpdate<<<1, 1, 0, st>>>(s.ctl, dt_max, max_disp);
+}
 
 InterView interView(const DevState& s, bool mixed) {
   InterView v;

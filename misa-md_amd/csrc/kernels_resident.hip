@@ -45,13 +45,14 @@ __global__ __launch_bounds__(kBlock) void kFirst(Geom G, CXPtr x, XPtr xn, const
   const int8_t ti = type[s];
   double y0 = x.c[0][s], y1 = x.c[1][s], y2 = x.c[2][s];
   if (!isVacant(ti)) {
-    const double c = vp.c[species(ti)];
+    const StepDt sd = stepDt(vp);
+    const double c = sd.c[species(ti)];
     double v0 = v.c[0][s] + c * f.c[0][s];
     double v1 = v.c[1][s] + c * f.c[1][s];
     double v2 = v.c[2][s] + c * f.c[2][s];
-    y0 = y0 + vp.dt * v0;
-    y1 = y1 + vp.dt * v1;
-    y2 = y2 + vp.dt * v2;
+    y0 = y0 + sd.dt * v0;
+    y1 = y1 + sd.dt * v1;
+    y2 = y2 + sd.dt * v2;
     if (leftSite(G, p, cx, cy, cz, y0, y1, y2)) {
       stageInter(st, ctl, refIndex(G, p, cx, cy, cz), s, id[s], ti, y0, y1, y2, v0, v1, v2);
       v0 = v1 = v2 = 0.0;

@@ -80,6 +80,9 @@ constexpr int kInterBlocks = 64;
 struct VerletParams {
   double dt;
   double c[3];  // 0.5 dt ftm2v / m per species (newton_motion.cpp:19-28)
+  // adaptive time step: the step length lives in device memory (Ctl::dt) and
+  // dt, c are derived from it in the kernels; null for a fixed step
+  const double* dtp = nullptr;
 };
 
 // Host launchers (kernels_resident.hip).
@@ -98,6 +101,9 @@ int64_t launchThermo(const DevState& s, hipStream_t st);
 // Partial sums per block: [e_embed, e_pair, virial] (3 per block)
 int64_t launchEnergy(const DevState& s, const LatOffsets& o, const Tables<double>& t64, hipStream_t st);
 void launchScaleV(const DevState& s, double factor, hipStream_t st);
+// Adaptive step: max |v|^2 over lattice and inter atoms into Ctl::vmax2, then
+// Ctl::dt = min(dt_max, max_disp / |v|max) and Ctl::t_sim += Ctl::dt.
+void launchAdaptiveDt(const DevState& s, double dt_max, double max_disp, hipStream_t st);
 // Inter-atom work (kernels_inter.hip).
 // decide mode 0: the step's ownership pass (append new inter atoms, vacancy
 // recombination, migration, ghost images, index, flags); mode 1: migration,

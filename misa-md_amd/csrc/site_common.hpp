@@ -123,6 +123,27 @@ __device__ __forceinline__ float4 siteRec(const Geom& G, int p, int cx, int cy, 
   return packRec(x0 - X, x1 - Y, x2 - Z, sp, 0.0f);
 }
 
+// Step length and half-kick factors of the current step: the fixed values,
+// or those of the adaptive step in device memory, computed as
+// NewtonMotion::preComputeDtInv2m does (newton_motion.cpp:19-28).
+struct StepDt {
+  double dt;
+  double c[3];
+};
+__device__ __forceinline__ StepDt stepDt(const VerletParams& vp) {
+#pragma clang fp contract(off)
+  StepDt r;
+  if (!vp.dtp) {
+    r.dt = vp.dt;
+    for (int t = 0; t < 3; t++) r.c[t] = vp.c[t];
+    return r;
+  }
+  r.dt = *vp.dtp;
+  const double h = 0.5 * r.dt * kFtm2vDev;
+  for (int t = 0; t < 3; t++) r.c[t] = h / speciesMass(t);
+  return r;
+}
+
 // Reference linear index of sub-box cell (cx, cy, cz) of parity p.
 __device__ __forceinline__ int64_t refIndex(const Geom& G, int p, int cx, int cy, int cz) {
   const int64_t sx = 2 * static_cast<int64_t>(G.NX);

@@ -69,6 +69,16 @@ _SIGS = {
     "misa_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "misa_set_timestep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     "misa_set_inter_capacity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "misa_set_adaptive_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    "misa_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "misa_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "misa_batch_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "misa_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
+    "misa_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
+                                     ctypes.POINTER(ctypes.c_double)]),
+    "misa_sim_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double)]),
     "misa_compute_forces": (ctypes.c_int, [ctypes.c_void_p]),
     "misa_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
     "misa_setv": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double),
@@ -216,6 +226,19 @@ class EamDevice:
     def set_inter_capacity(self, cap):
         self._check(self.lib.misa_set_inter_capacity(self.h, int(cap)), "misa_set_inter_capacity")
 
+    def set_adaptive_dt(self, dt_max, max_disp):
+        self._check(self.lib.misa_set_adaptive_dt(self.h, dt_max, max_disp), "misa_set_adaptive_dt")
+
+    def sim_time(self):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.misa_sim_time(self.h, ctypes.byref(t), ctypes.byref(dt)), "misa_sim_time")
+        return t.value, dt.value
+
+    def autotune(self, steps):
+        ty, ms = ctypes.c_int32(), ctypes.c_double()
+        self._check(self.lib.misa_autotune(self.h, int(steps), ctypes.byref(ty), ctypes.byref(ms)), "misa_autotune")
+        return ty.value, ms.value
+
     def set_precision(self, mode):
         self._check(self.lib.misa_set_precision(self.h, mode), "misa_set_precision")
 
@@ -256,6 +279,38 @@ class EamDevice:
         self._check(self.lib.misa_last_launch_counts(self.h, cnt), "misa_last_launch_counts")
         return {"rho_df_ms": out[0], "force_ms": out[1], "other_ms": out[2], "total_ms": out[3],
                 "launches": list(cnt)}
+
+
+class EamBatch:
+    """Independent boxes stepped together through captured step graphs."""
+
+    def __init__(self, devices):
+        self.lib = load()
+        self.devices = list(devices)  # keeps the handles alive
+        arr = (ctypes.c_void_p * len(self.devices))(*[d.h.value for d in self.devices])
+        b = ctypes.c_void_p()
+        rc = self.lib.misa_batch_create(arr, len(self.devices), ctypes.byref(b))
+        if rc != 0:
+            raise MisaError("misa_batch_create failed (%d)" % rc)
+        self.b = b
+
+    def step(self, n):
+        r = ctypes.c_int64()
+        rc = self.lib.misa_batch_step(self.b, int(n), ctypes.byref(r))
+        if rc != 0:
+            raise MisaError("misa_batch_step failed (%d): %s" % (rc, self.lib.misa_batch_last_error(self.b).decode()))
+        return r.value
+
+    def close(self):
+        if self.b is not None:
+            self.lib.misa_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def geom(cells, lattice_const=A_FE, rc_factor=RC_FACTOR):

@@ -102,6 +102,9 @@ void oracle_inter_ghost_get(void* h, AtomElement* out) {
 }
 
 double oracle_mvv(void* h) { return static_cast<Sim*>(h)->mvv(); }
+double oracle_adaptive_dt(void* h, double dt_max, double max_disp) {
+  return static_cast<Sim*>(h)->adaptiveDt(dt_max, max_disp);
+}
 double oracle_temperature(void* h) { return static_cast<Sim*>(h)->temperature(); }
 void oracle_rescale(void* h, double T) { static_cast<Sim*>(h)->rescale(T); }
 void oracle_energy(void* h, double* out) { static_cast<Sim*>(h)->energy(out); }

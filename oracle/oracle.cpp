@@ -417,6 +417,25 @@ double Sim::temperature() {  // :51-59
   return mvv() * kMvv2e / (dof * kBoltz);
 }
 
+double Sim::adaptiveDt(double dt_max, double max_disp) {
+  double vmax2 = 0.0;
+  auto visit = [&](const AtomElement& at) {
+    const double v2 = (at.v[0] * at.v[0] + at.v[1] * at.v[1]) + at.v[2] * at.v[2];
+    if (v2 > vmax2) vmax2 = v2;
+  };
+  for (int64_t k = 0; k < d.dbx_sub[2]; k++)
+    for (int64_t j = 0; j < d.dbx_sub[1]; j++)
+      for (int64_t i = 0; i < d.dbx_sub[0]; i++) {
+        AtomElement& at = subBox(i, j, k);
+        if (at.type != kInvalid) visit(at);
+      }
+  for (auto& at : inter_list) visit(at);
+  const double vmax = std::sqrt(vmax2);
+  const double dt_ = vmax > 0.0 ? std::fmin(dt_max, max_disp / vmax) : dt_max;
+  setDt(dt_);
+  return dt_;
+}
+
 void Sim::rescale(double T) {  // :86-111
   const double factor = std::sqrt(T / temperature());
   for (int64_t k = 0; k < d.dbx_sub[2]; k++)

@@ -193,6 +193,9 @@ struct Sim {
   // system configuration (system_configuration.cpp:26-126)
   double mvv();
   double temperature();
+  // Adaptive step length (no reference counterpart): dt = min(dt_max,
+  // max_disp / max|v|) over lattice and inter atoms, then setDt(dt).
+  double adaptiveDt(double dt_max, double max_disp);
   void rescale(double T);
   int64_t globalAtoms() const { return 2 * d.n[0] * d.n[1] * d.n[2]; }
   // potential energy and virial (no reference counterpart)

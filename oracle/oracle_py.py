@@ -54,6 +54,7 @@ def load():
             "oracle_inter_ghost_count": (i64, [vp]),
             "oracle_inter_ghost_get": (None, [vp, vp]),
             "oracle_mvv": (d, [vp]),
+            "oracle_adaptive_dt": (d, [vp, d, d]),
             "oracle_temperature": (d, [vp]),
             "oracle_rescale": (None, [vp, d]),
             "oracle_energy": (None, [vp, vp]),
@@ -227,6 +228,10 @@ class Sim:
         if n:
             self.L.oracle_inter_ghost_get(self.h, _p(out))
         return out
+
+    def adaptive_dt(self, dt_max, max_disp):
+        """Set dt = min(dt_max, max_disp / max|v|) for the next step; returns it."""
+        return self.L.oracle_adaptive_dt(self.h, dt_max, max_disp)
 
     def mvv(self):
         return self.L.oracle_mvv(self.h)

@@ -242,3 +242,59 @@ def test_far_lattice_atoms_use_the_full_list(setfl_fecuni, mode):
         assert abs(t["e_potential"] - e["e_potential"]) / (2 * np.prod(cells)) < 1e-13
     else:
         assert rms < 1e-5
+
+
+def test_adaptive_dt_cascade_matches_oracle(setfl_fecuni):
+    """A 5 keV recoil along [1 3 5] with dt = min(1 fs, 0.01 A / max|v|) taken
+    every step on each side from its own velocities."""
+    cells = (10, 10, 10)
+    sim = O.Sim(cells, A, F, setfl_fecuni)
+    sim.build_world(466953, 300.0, (985, 10, 5), 1)
+    dev = _device(setfl_fecuni, cells)
+    dev.upload(sim.atoms().copy())
+    dev.set_timestep(0.001)
+    dev.compute_forces()
+    sim.prepare()
+    lat, direction = (5, 5, 5, 0), (1.0, 3.0, 5.0)
+    dev.collision_step(lat, direction, 5000.0)
+    sim.collision(lat, direction, 5000.0)
+    dev.set_adaptive_dt(0.001, 0.01)
+    t_oracle = 0.0
+    for k in range(4):
+        for _ in range(25):
+            t_oracle += sim.adaptive_dt(0.001, 0.01)
+            sim.run(1)
+        dev.step(25)
+        t_dev, dt_dev = dev.sim_time()
+        say("adaptive +%d steps: t %.6e ps (oracle %.6e), last dt %.3e" % (25 * (k + 1), t_dev, t_oracle, dt_dev))
+        assert t_dev == pytest.approx(t_oracle, rel=1e-12)
+        _compare_state(dev, sim, 1e-9, 1e-5, 1e-8, "adaptive +%d steps" % (25 * (k + 1)))
+    assert dt_dev < 2e-5  # the recoil sets the step
+
+
+def test_four_simultaneous_recoils(setfl_fecuni):
+    """Several PKAs in one stage: setv per recoil, then one force evaluation."""
+    cells = (10, 10, 10)
+    sim = O.Sim(cells, A, F, setfl_fecuni)
+    sim.build_world(2024, 300.0, (985, 10, 5), 1)
+    dev = _device(setfl_fecuni, cells)
+    dev.upload(sim.atoms().copy())
+    dev.set_timestep(0.0002)
+    dev.compute_forces()
+    sim.set_dt(0.0002)
+    sim.prepare()
+    rng = np.random.default_rng(4)
+    pkas = []
+    for q in range(4):
+        lat = (int(rng.integers(10)), int(rng.integers(10)), int(rng.integers(10)), int(rng.integers(2)))
+        u = rng.normal(size=3)
+        pkas.append((lat, tuple(u / np.linalg.norm(u))))
+    for lat, d in pkas[:-1]:
+        dev.setv(lat, d, 500.0)
+        sim.setv(lat, d, 500.0)
+    dev.collision_step(pkas[-1][0], pkas[-1][1], 500.0)
+    sim.collision(pkas[-1][0], pkas[-1][1], 500.0)
+    for k in range(3):
+        dev.step(20)
+        sim.run(20)
+        _compare_state(dev, sim, 1e-9, 1e-6, 1e-8, "4 recoils +%d steps" % (20 * (k + 1)))

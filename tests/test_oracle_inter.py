@@ -137,3 +137,34 @@ def test_cascade_energy_conservation(setfl_fe):
         e.append(etot())
     assert seen >= 2
     assert np.ptp(e) < 2e-4
+
+
+def test_adaptive_dt_bounds_the_displacement(setfl_fe):
+    """dt = min(dt_max, d / max|v|): the step the oracle takes equals the
+    formula on its own velocities and keeps every displacement near d."""
+    cells = (6, 6, 6)
+    sim = O.Sim(cells, A, F, setfl_fe)
+    sim.build_world(466953, 300.0, (1, 0, 0))
+    sim.prepare()
+    sim.collision([3, 3, 3, 0], [1.0, 3.0, 5.0], 5000.0)
+    for _ in range(30):
+        sb = sim.sub_box()
+        live = sb["type"] >= 0
+        inter = sim.inter()
+        v = np.concatenate([sb["v"][live].reshape(-1, 3), inter["v"].reshape(-1, 3)])
+        vmax = np.sqrt(np.max(np.sum(v * v, axis=1)))
+        x0 = np.concatenate([sb["x"][live].reshape(-1, 3), inter["x"].reshape(-1, 3)])
+        ids0 = np.concatenate([sb["id"][live], inter["id"]])
+        dt = sim.adaptive_dt(0.001, 0.01)
+        assert dt == pytest.approx(min(0.001, 0.01 / vmax), rel=1e-15)
+        sim.run(1)
+        sb = sim.sub_box()
+        live = sb["type"] >= 0
+        inter = sim.inter()
+        x1 = np.concatenate([sb["x"][live].reshape(-1, 3), inter["x"].reshape(-1, 3)])
+        ids1 = np.concatenate([sb["id"][live], inter["id"]])
+        o0, o1 = np.argsort(ids0), np.argsort(ids1)
+        d = x1[o1] - x0[o0]
+        L = np.array(cells) * A
+        d -= L * np.round(d / L)
+        assert np.max(np.linalg.norm(d, axis=1)) < 0.0101
