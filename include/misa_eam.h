@@ -50,6 +50,8 @@ typedef struct misa_batch misa_batch;
 /* precision of the resident-mode force path */
 #define MISA_PREC_FP64 0     /* FP64 everywhere */
 #define MISA_PREC_MIXED 1    /* FP32 splines and pair terms, FP64 positions/Verlet/sums */
+#define MISA_PREC_MIXED_KAHAN 17   /* MIXED with per-atom sums in FP32 plus Kahan compensation */
+#define MISA_PREC_MIXED_F32SUM 33  /* MIXED with plain FP32 per-atom sums */
 
 /* Geometry of the single-rank domain: the whole periodic box.  Mirrors the
  * comm::BccDomain fields the reference reads (simulation.cpp:29-56):
@@ -193,10 +195,15 @@ const char* misa_batch_last_error(const misa_batch* b);
 /* `nsteps` loop bodies of every box, as misa_step would run them one box at
  * a time; `graph_replays` (may be NULL) returns the replays so far. */
 int misa_batch_step(misa_batch* b, int64_t nsteps, int64_t* graph_replays);
-/* Autotuner of the site kernels' tile shape: times `steps` steps for every
- * y-tile height dividing the box (up to 32) and keeps the fastest; the trial
- * steps advance the simulation. */
+/* Autotuner of the site kernels' launch shape: times `steps` steps for every
+ * y-tile height dividing the box (up to 32) and, in mixed precision, every
+ * count of neighbour gathers in flight per wave (4 or 8 in the density pass,
+ * 2 or 4 in the force pass), and keeps the fastest; the trial steps advance
+ * the simulation.  The arithmetic of a site does not depend on the shape. */
 int misa_autotune(misa_handle* h, int64_t steps, int32_t* best_ty, double* best_ms);
+/* Set / read the launch shape directly (y-tile rows, gathers in flight). */
+int misa_set_launch_shape(misa_handle* h, int32_t ty, int32_t gathers_rho, int32_t gathers_force);
+int misa_get_launch_shape(const misa_handle* h, int32_t* ty, int32_t* gathers_rho, int32_t* gathers_force);
 
 /* ---- host-only set-up: no device access, usable without a GPU ---- */
 

@@ -334,6 +334,7 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
   if (rc == MISA_OK) rc |= devAlloc(h, &s.rec[0], slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.rec[1], slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.dl, 2 * kMaxLatOffsets);
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.wmask, static_cast<size_t>(2 * 4 * 16 * G.W.bpx));
   if (rc == MISA_OK) rc |= devAlloc(h, &s.type, slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.id, slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.ctl, 1);
@@ -449,8 +450,9 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
   const int64_t sF = static_cast<int64_t>(sf.nrho) * kKnot, sR = static_cast<int64_t>(sf.nr) * kKnot;
   const int64_t nr = sf.nr;
   const int64_t oRhoV = 3 * sF + 9 * sR;            // [3][nr][4] density value cubics
-  const int64_t oPair = oRhoV + 3 * nr * kRhoRec;  // [9][nr][16] force records per ordered pair
-  const int64_t total = oPair + 9 * nr * kPairRec;
+  const int64_t oPair = oRhoV + 3 * nr * kRhoRec;  // [9][nr][8] force records per ordered pair
+  const int64_t oRhoD = oPair + 9 * nr * kPairRec;  // [3][nr][4] density slope cubics (FP32 force pass)
+  const int64_t total = oRhoD + 3 * nr * kRhoRec;
   std::vector<double> h64(static_cast<size_t>(total), 0.0);
   auto put = [&](int64_t base, const std::vector<double>& c7, int n) {
     for (int m = 0; m < n; m++) {
@@ -500,6 +502,24 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
     }
   std::vector<float> h32(h64.size());
   for (size_t i = 0; i < h64.size(); i++) h32[i] = static_cast<float>(h64[i]);
+  // FP32 force records: the density half holds the slope cubic divided by the
+  // knot spacing, so that a slope is two FMAs in the force pass
+  const double rdr = 1.0 / sf.dr;
+  auto slope = [&](const std::vector<double>& R, int64_t m, float* out) {
+    out[0] = static_cast<float>(3.0 * R[m * 7 + 3] * rdr);
+    out[1] = static_cast<float>(2.0 * R[m * 7 + 4] * rdr);
+    out[2] = static_cast<float>(R[m * 7 + 5] * rdr);
+    out[3] = 0.0f;
+  };
+  for (int a = 0; a < 3; a++)
+    for (int b2 = 0; b2 < 3; b2++) {
+      if (el[a] < 0 || el[b2] < 0) continue;
+      for (int64_t m = 0; m < nr; m++) slope(h->cR[el[b2]], m, &h32[oPair + ((a * 3 + b2) * nr + m) * kPairRec + 4]);
+    }
+  for (int t = 0; t < 3; t++) {
+    if (el[t] < 0) continue;
+    for (int64_t m = 0; m < nr; m++) slope(h->cR[el[t]], m, &h32[oRhoD + (t * nr + m) * kRhoRec]);
+  }
   if (!h->tab64) {
     if (devAlloc(h, &h->tab64, static_cast<size_t>(total)) != MISA_OK) return MISA_E_HIP;
     if (devAlloc(h, &h->tab32, static_cast<size_t>(total)) != MISA_OK) return MISA_E_HIP;
@@ -521,6 +541,7 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
   a64.rhomax = (sf.nrho - 1) * sf.drho;
   a64.rhoV = h->tab64 + oRhoV;
   a64.pairF = h->tab64 + oPair;
+  a64.rhoD = h->tab64 + oRhoD;
   Tables<float>& a32 = h->t32;
   a32.F = h->tab32;
   a32.rho = h->tab32 + 3 * sF;
@@ -534,6 +555,7 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
   a32.rhomax = static_cast<float>(a64.rhomax);
   a32.rhoV = h->tab32 + oRhoV;
   a32.pairF = h->tab32 + oPair;
+  a32.rhoD = h->tab32 + oRhoD;
   h->have_pot = true;
   return MISA_OK;
 }
@@ -660,9 +682,12 @@ int misa_set_precision(misa_handle* h, int mode) {
   } else if (mode == MISA_PREC_MIXED) {
     h->precision = mode;
     h->acc = 0;
-  } else if (mode == MISA_PREC_MIXED + 16) {  // mixed with FP32 Kahan-compensated per-atom sums
+  } else if (mode == MISA_PREC_MIXED_KAHAN) {
     h->precision = MISA_PREC_MIXED;
     h->acc = 1;
+  } else if (mode == MISA_PREC_MIXED_F32SUM) {
+    h->precision = MISA_PREC_MIXED;
+    h->acc = 2;
   } else {
     return fail(h, MISA_E_ARG, "unknown precision mode");
   }
@@ -1144,20 +1169,38 @@ int misa_batch_step(misa_batch* b, int64_t nsteps, int64_t* graph_replays) {
   return rc;
 }
 
-// Tile shape of the site kernels: try every y-tile height that divides the
-// box (up to 32 rows) over `steps` steps each and keep the fastest.  The trial
-// steps are real steps of the simulation.
+// Launch shape of the site kernels: every y-tile height that divides the
+// box (up to 32 rows) and, in mixed precision, the gathers each wave keeps in
+// flight in the density pass (4 or 8) and in the force pass (2 or 4).  Each
+// candidate runs `steps` steps (real steps of the simulation); the fastest
+// is kept.  Only the assignment of sites to threads and the grouping of the
+// loads change, not the arithmetic of a site.
 int misa_autotune(misa_handle* h, int64_t steps, int32_t* best_ty, double* best_ms) {
   if (!h || steps < 1) return MISA_E_ARG;
   if (!h->forces_ready) return fail(h, MISA_E_ARG, "misa_compute_forces must run before misa_autotune");
   HIPCHECK(h, hipSetDevice(h->device));
   const bool prof = h->profile;
   h->profile = false;
-  int bestTy = h->s.G.W.ty;
-  double bestMs = 1e300;
+  struct Shape {
+    int ty, uRho, uForce;
+  };
+  std::vector<Shape> cands;
+  const bool mixed = h->precision == MISA_PREC_MIXED;
   for (int ty = 1; ty <= std::min(32, h->s.G.n1); ty++) {
     if (h->s.G.n1 % ty) continue;
-    h->s.G.W.ty = ty;
+    if (!mixed) {
+      cands.push_back({ty, 4, 4});
+      continue;
+    }
+    for (int ur : {4, 8})
+      for (int uf : {2, 4}) cands.push_back({ty, ur, uf});
+  }
+  Shape best{h->s.G.W.ty, h->s.uRho, h->s.uForce};
+  double bestMs = 1e300;
+  for (const Shape& c : cands) {
+    h->s.G.W.ty = c.ty;
+    h->s.uRho = c.uRho;
+    h->s.uForce = c.uForce;
     h->gen++;
     const int rc = misa_step(h, steps);
     if (rc != MISA_OK) {
@@ -1167,14 +1210,36 @@ int misa_autotune(misa_handle* h, int64_t steps, int32_t* best_ty, double* best_
     const double ms = h->timings[3] / static_cast<double>(steps);
     if (ms < bestMs) {
       bestMs = ms;
-      bestTy = ty;
+      best = c;
     }
   }
-  h->s.G.W.ty = bestTy;
+  h->s.G.W.ty = best.ty;
+  h->s.uRho = best.uRho;
+  h->s.uForce = best.uForce;
   h->gen++;
   h->profile = prof;
-  if (best_ty) *best_ty = bestTy;
+  if (best_ty) *best_ty = best.ty;
   if (best_ms) *best_ms = bestMs;
+  return MISA_OK;
+}
+
+int misa_set_launch_shape(misa_handle* h, int32_t ty, int32_t gathers_rho, int32_t gathers_force) {
+  if (!h) return MISA_E_ARG;
+  if (ty < 1 || h->s.G.n1 % ty || (gathers_rho != 4 && gathers_rho != 8) ||
+      (gathers_force != 2 && gathers_force != 4))
+    return fail(h, MISA_E_ARG, "y-tile must divide the box; gathers 4|8 (density), 2|4 (force)");
+  h->s.G.W.ty = ty;
+  h->s.uRho = gathers_rho;
+  h->s.uForce = gathers_force;
+  h->gen++;
+  return MISA_OK;
+}
+
+int misa_get_launch_shape(const misa_handle* h, int32_t* ty, int32_t* gathers_rho, int32_t* gathers_force) {
+  if (!h || !ty || !gathers_rho || !gathers_force) return MISA_E_ARG;
+  *ty = h->s.G.W.ty;
+  *gathers_rho = h->s.uRho;
+  *gathers_force = h->s.uForce;
   return MISA_OK;
 }
 

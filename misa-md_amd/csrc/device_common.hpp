@@ -107,7 +107,11 @@ struct Tables {
   T rhomax;
   // packed records for the resident kernels
   const T* rhoV;   // [3][nr][4]: value cubic c3..c6 of rho_e
-  const T* pairF;  // [9][nr][8]: value cubics of (r phi)_eiej and rho_ej (c3..c6 each)
+  // [9][nr][8]: FP64: value cubics of (r phi)_eiej and rho_ej (c3..c6 each);
+  // FP32: value cubic of (r phi)_eiej, then the slope cubic of rho_ej in the
+  // knot coordinate already divided by the knot spacing (3 c3/dr, 2 c4/dr, c5/dr)
+  const T* pairF;
+  const T* rhoD;   // [3][nr][4]: slope cubic of rho_e as in pairF (FP32 force pass)
 };
 constexpr int kPairRec = 8;
 constexpr int kRhoRec = 4;
@@ -115,6 +119,18 @@ constexpr int kRhoRec = 4;
 __device__ __forceinline__ bool isVacant(int8_t t) { return t < 0; }
 __device__ __forceinline__ int species(int8_t t) { return t & 3; }
 __device__ __forceinline__ int pairIdx(int a, int b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; }
+
+// Knot search of the mixed-precision passes for r below the cutoff: knot
+// floor(r/dr) (at most n-2) and the local coordinate r/dr - knot (at most 1).
+// The same knot and coordinate as knotOf in exact arithmetic, without the
+// one-based detour.
+__device__ __forceinline__ int knotM(float r, float rdr, int n, float& q) {
+#pragma clang fp contract(off)
+  const float t = r * rdr;
+  const int m = min(static_cast<int>(t), n - 2);
+  q = fminf(t - static_cast<float>(m), 1.0f);
+  return m;
+}
 
 // Knot search of the eam/alloy convention: p = x/delta + 1 (one-based knot),
 // m = int(p) clamped to n-1 (and to >= 1 for rho-tables), p -= m, p <= 1.

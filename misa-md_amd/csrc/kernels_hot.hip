@@ -37,72 +37,14 @@
 #include "inter_common.hpp"
 #include "resident.hpp"
 #include "site_common.hpp"
+#include "buffer_load.hpp"
+#include "mixed_site.hpp"
 
 namespace misa {
 
 namespace {
 
 using namespace site;
-
-typedef __amdgpu_buffer_rsrc_t Rsrc;
-
-__device__ __forceinline__ Rsrc rsrcAt(uintptr_t base) {
-  return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(base), 0, 0x7fffffff, 0x00020000);
-}
-// Resource whose base sits `lo` elements (lo <= 0) below array `p`.
-template <typename T>
-__device__ __forceinline__ Rsrc rsrcBelow(const T* p, int lo) {
-  return rsrcAt(reinterpret_cast<uintptr_t>(p) + static_cast<intptr_t>(lo) * static_cast<intptr_t>(sizeof(T)));
-}
-
-__device__ __forceinline__ double ldD(Rsrc r, uint32_t v, uint32_t s) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
-}
-__device__ __forceinline__ float ldF(Rsrc r, uint32_t v, uint32_t s) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, 0));
-}
-__device__ __forceinline__ int8_t ldB(Rsrc r, uint32_t v, uint32_t s) {
-  return static_cast<int8_t>(__builtin_amdgcn_raw_buffer_load_b8(r, v, s, 0));
-}
-template <typename T>
-__device__ __forceinline__ T ldT(Rsrc r, uint32_t v, uint32_t s);
-template <>
-__device__ __forceinline__ double ldT<double>(Rsrc r, uint32_t v, uint32_t s) {
-  return ldD(r, v, s);
-}
-template <>
-__device__ __forceinline__ float ldT<float>(Rsrc r, uint32_t v, uint32_t s) {
-  return ldF(r, v, s);
-}
-
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-
-// First N values of a record at byte offset `o` of resource `r`.  The 16-byte
-// load is reinterpreted as a whole (float4 / double2): bit-casting single
-// elements of the loaded vector is miscompiled by this toolchain (only lane 0
-// is kept).
-template <typename T, int N>
-__device__ __forceinline__ void ldRec(Rsrc r, uint32_t o, T (&out)[N]) {
-  constexpr int per = 16 / sizeof(T);
-  constexpr int full = N / per;
-#pragma unroll
-  for (int q = 0; q < full; q++) {
-    const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(r, o + 16 * q, 0, 0);
-    if constexpr (sizeof(T) == 4) {
-      const float4 f = __builtin_bit_cast(float4, w);
-      out[4 * q + 0] = f.x;
-      out[4 * q + 1] = f.y;
-      out[4 * q + 2] = f.z;
-      out[4 * q + 3] = f.w;
-    } else {
-      const double2 d = __builtin_bit_cast(double2, w);
-      out[2 * q + 0] = d.x;
-      out[2 * q + 1] = d.y;
-    }
-  }
-#pragma unroll
-  for (int q = full * per; q < N; q++) out[q] = ldT<T>(r, o + q * static_cast<uint32_t>(sizeof(T)), 0);
-}
 
 __device__ __forceinline__ double rsqT(double x) { return rsqrtRefined(x); }
 __device__ __forceinline__ float rsqT(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -200,6 +142,57 @@ __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T>
   } else {
     forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) { df[q] = dfi; });
   }
+}
+
+// secondstep (and, fused, the next firststep and decide) of a lattice atom,
+// newton_motion.cpp:46-77, with every product and sum rounded separately.
+template <int MODE>
+__device__ __forceinline__ void verletTail(const Geom& G, int64_t s, int p, int cx, int cy, int cz, int8_t ti,
+                                           int ei, double xi0, double xi1, double xi2, double f0, double f1,
+                                           double f2, XPtr xn, XPtr v, XPtr f, const uint64_t* __restrict__ id,
+                                           const VerletParams& vp, const StageArgs& st, Ctl* ctl) {
+#pragma clang fp contract(off)
+  const StepDt sd = stepDt(vp, ei);
+  const double c = sd.c;
+  double v0 = v.c[0][s] + c * f0;
+  double v1 = v.c[1][s] + c * f1;
+  double v2 = v.c[2][s] + c * f2;
+  if (MODE == 2) {
+    f.c[0][s] = f0;
+    f.c[1][s] = f1;
+    f.c[2][s] = f2;
+    return;
+  }
+  if (MODE == 0) {
+    f.c[0][s] = f0;
+    f.c[1][s] = f1;
+    f.c[2][s] = f2;
+    v.c[0][s] = v0;
+    v.c[1][s] = v1;
+    v.c[2][s] = v2;
+    return;
+  }
+  v0 = v0 + c * f0;
+  v1 = v1 + c * f1;
+  v2 = v2 + c * f2;
+  const double y0 = xi0 + sd.dt * v0;
+  const double y1 = xi1 + sd.dt * v1;
+  const double y2 = xi2 + sd.dt * v2;
+  if (leftSite(G, p, cx, cy, cz, y0, y1, y2)) {
+    stageInter(st, ctl, refIndex(G, p, cx, cy, cz), s, id[s], ti, y0, y1, y2, v0, v1, v2);
+    v0 = v1 = v2 = 0.0;
+  }
+  v.c[0][s] = v0;
+  v.c[1][s] = v1;
+  v.c[2][s] = v2;
+  xn.c[0][s] = y0;
+  xn.c[1][s] = y1;
+  xn.c[2][s] = y2;
+  forEachImage(G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
+    xn.c[0][q] = y0 + sx * G.L0;
+    xn.c[1][q] = y1 + sy * G.L1;
+    xn.c[2][q] = y2 + sz * G.L2;
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -338,47 +331,7 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
   }
   double f0, f1, f2;
   sum3Value<ACC>(acc, f0, f1, f2);
-  const StepDt sd = stepDt(vp);
-  const double c = sd.c[ei];
-  double v0 = v.c[0][s] + c * f0;
-  double v1 = v.c[1][s] + c * f1;
-  double v2 = v.c[2][s] + c * f2;
-  if (MODE == 2) {
-    f.c[0][s] = f0;
-    f.c[1][s] = f1;
-    f.c[2][s] = f2;
-    return;
-  }
-  if (MODE == 0) {
-    f.c[0][s] = f0;
-    f.c[1][s] = f1;
-    f.c[2][s] = f2;
-    v.c[0][s] = v0;
-    v.c[1][s] = v1;
-    v.c[2][s] = v2;
-    return;
-  }
-  v0 = v0 + c * f0;
-  v1 = v1 + c * f1;
-  v2 = v2 + c * f2;
-  const double y0 = xi0 + sd.dt * v0;
-  const double y1 = xi1 + sd.dt * v1;
-  const double y2 = xi2 + sd.dt * v2;
-  if (leftSite(G, p, cx, cy, cz, y0, y1, y2)) {
-    stageInter(st, ctl, refIndex(G, p, cx, cy, cz), s, id[s], ti, y0, y1, y2, v0, v1, v2);
-    v0 = v1 = v2 = 0.0;
-  }
-  v.c[0][s] = v0;
-  v.c[1][s] = v1;
-  v.c[2][s] = v2;
-  xn.c[0][s] = y0;
-  xn.c[1][s] = y1;
-  xn.c[2][s] = y2;
-  forEachImage(G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
-    xn.c[0][q] = y0 + sx * G.L0;
-    xn.c[1][q] = y1 + sy * G.L1;
-    xn.c[2][q] = y2 + sz * G.L2;
-  });
+  verletTail<MODE>(G, s, p, cx, cy, cz, ti, ei, xi0, xi1, xi2, f0, f1, f2, xn, v, f, id, vp, st, ctl);
 }
 
 // ---------------------------------------------------------------------------
@@ -390,21 +343,23 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
 //   pairs within that distance of the cutoff, where the tabulated functions
 //   and their slopes vanish.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float4 ldRec4(Rsrc r, uint32_t v, uint32_t s) {
-  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, 0));
-}
+// Offsets of the pruned list (at most 128) within the cutoff for at least
+// one lane of a wavefront, recorded by the density pass for the force pass,
+// which then skips the others: both passes see the same positions and round
+// the distances the same way, so the skipped offsets have no pair.
+__device__ __forceinline__ uint32_t waveId() { return blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); }
 
 template <int ACC, int U>
 __global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<float> tb, CXPtr x,
                                                   const int8_t* __restrict__ type, float4* rec,
                                                   const float4* __restrict__ dl, double* __restrict__ rho_out,
-                                                  InterView iv) {
+                                                  uint64_t* __restrict__ wmask, InterView iv) {
   int64_t s;
   int p, cx, cy, cz;
   if (!siteOfThread(G, s, p, cx, cy, cz)) return;
   const int8_t ti = type[s];
-  float dfi = 0.0f;
-  double rho = 0.0;
+  Sum1<ACC> acc;
+  uint64_t w0 = 0, w1 = 0;  // offsets in range for some lane (uniform among the lanes that gather)
   if (!isVacant(ti)) {
     const float4 me = rec[s];
     const Rsrc rr = rsrcBelow(rec, O.lo);
@@ -414,100 +369,54 @@ __global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<f
     const float4* D = dl + p * kMaxLatOffsets;
     const float rc2 = static_cast<float>(G.rc2);
     const int nk = O.n[p];
-    Sum1<ACC> acc;
     auto group = [&](auto UC, int k0) {
       constexpr int V = decltype(UC)::value;
-      float4 nb[V];
-#pragma unroll
-      for (int u = 0; u < V; ++u) nb[u] = ldRec4(rr, o16, sh[k0 + u] * 16u);
+      float4 nb[V], d[V];
       bool in[V];
-      float pp[V];
-      float cr[V][kRhoRec];
 #pragma unroll
       for (int u = 0; u < V; ++u) {
-        const float4 d = D[k0 + u];
-        const float d0 = (me.x - nb[u].x) - d.x, d1 = (me.y - nb[u].y) - d.y, d2 = (me.z - nb[u].z) - d.z;
-        const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        in[u] = r2 < rc2;
-        if (in[u]) {
-          const int m = knotOf(r2 * __builtin_amdgcn_rsqf(r2), tb.rdr, tb.nr, false, pp[u]);
-          const uint32_t off = static_cast<uint32_t>(recSpecies(nb[u].x) * tb.nr + m) *
-                               static_cast<uint32_t>(kRhoRec * sizeof(float));
-          ldRec<float, kRhoRec>(rtab, off, cr[u]);
-        }
+        nb[u] = ldRec4(rr, o16, sh[k0 + u] * 16u);
+        d[u] = D[k0 + u];
       }
+      rhoGroupM<ACC, V>(me, nb, d, rc2, rtab, tb, acc, in);
 #pragma unroll
       for (int u = 0; u < V; ++u)
-        if (in[u]) acc.add(((cr[u][0] * pp[u] + cr[u][1]) * pp[u] + cr[u][2]) * pp[u] + cr[u][3]);
+        if (__ballot(in[u])) {
+          const int k = k0 + u;
+          if (k < 64) w0 |= 1ull << k;
+          else w1 |= 1ull << (k - 64);
+        }
     };
     int k = 0;
     for (; k + U <= nk; k += U) group(std::integral_constant<int, U>{}, k);
     for (; k < nk; ++k) group(std::integral_constant<int, 1>{}, k);
-    if (ti & kNearInter) {  // inter atoms whose nearest site is within the full list (atom.cpp:224-240)
-      const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
-      const int ne = static_cast<int>(iv.ctl->n_inter + iv.ctl->n_ghost);
-      const FullPlaneOffsets& full = *iv.full;
-      auto bucket = [&](int64_t q) {
-        forBucket(iv, q, ne, [&](int e) {
-          const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
-          const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-          if (r2 < G.rc2) acc.add(rhoPair(tb, species(iv.type[iv.e_src[e]]), r2));
-        });
-      };
-      bucket(s);
-      for (int q = 0; q < full.n[p]; q++) bucket(s + full.off[p][q]);
-    }
-    if (ti & kWide) {  // full-list offsets beyond the pruned list, near a far lattice atom
-      const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
-      const FullPlaneOffsets& full = *iv.full;
-      for (int q = 0; q < full.nw[p]; q++) {
-        const int64_t j = s + full.woff[p][q];
-        const int8_t tj = type[j];
-        const double d0 = xi0 - x.c[0][j], d1 = xi1 - x.c[1][j], d2 = xi2 - x.c[2][j];
-        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        if (!isVacant(tj) && r2 < G.rc2) acc.add(rhoPair(tb, species(tj), r2));
-      }
-    }
-    rho = acc.value();
-    dfi = dEmbed(tb, species(ti), static_cast<float>(rho));
   }
-  float* w = reinterpret_cast<float*>(rec) + 3;
-  w[4 * s] = dfi;
-  if (rho_out) rho_out[s] = rho;
-  forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) {
-    w[4 * q] = dfi;
-    if (rho_out) rho_out[q] = rho;
-  });
+  // one lane that gathered writes the wavefront's offsets
+  const uint64_t gathered = __ballot(!isVacant(ti));
+  if (gathered) {
+    const int leader = __builtin_ctzll(gathered);
+    const uint64_t a0 = __shfl(w0, leader, 64), a1 = __shfl(w1, leader, 64);
+    if (static_cast<int>(threadIdx.x & 63) == leader) {
+      wmask[2 * waveId()] = a0;
+      wmask[2 * waveId() + 1] = a1;
+    }
+  }
+  rhoTailM<ACC>(G, s, p, cx, cy, cz, ti, acc, x, type, tb, iv, rec, rho_out);
 }
 
 template <int ACC, int MODE, int U>
 __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<float> tb, CXPtr x, XPtr xn,
                                                   const int8_t* __restrict__ type, const float4* __restrict__ rec,
                                                   float4* __restrict__ recn, const float4* __restrict__ dl,
-                                                  XPtr v, XPtr f, const uint64_t* __restrict__ id, VerletParams vp,
-                                                  StageArgs st, Ctl* ctl, InterView iv) {
+                                                  const uint64_t* __restrict__ wmask, XPtr v, XPtr f,
+                                                  const uint64_t* __restrict__ id, VerletParams vp, StageArgs st,
+                                                  Ctl* ctl, InterView iv) {
   int64_t s;
   int p, cx, cy, cz;
   if (!siteOfThread(G, s, p, cx, cy, cz)) return;
   const int8_t ti = type[s];
-  const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
   if (isVacant(ti)) {  // vacancy: keeps its stale position (atom.cpp:43-47)
-    if (MODE != 1) {   // clearForce left 0 there and latForce skips it (atom.cpp:86-100, 311-358)
-      f.c[0][s] = 0.0;
-      f.c[1][s] = 0.0;
-      f.c[2][s] = 0.0;
-    } else {
-      xn.c[0][s] = xi0;
-      xn.c[1][s] = xi1;
-      xn.c[2][s] = xi2;
-      recn[s] = vacantRec();
-      forEachImage(G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
-        xn.c[0][q] = xi0 + sx * G.L0;
-        xn.c[1][q] = xi1 + sy * G.L1;
-        xn.c[2][q] = xi2 + sz * G.L2;
-        recn[q] = vacantRec();
-      });
-    }
+    vacancyForceM<MODE>(G, s, p, cx, cy, cz, x, xn, recn, f);
     return;
   }
   const int ei = species(ti);
@@ -515,136 +424,51 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
   const float dfi = me.w;
   const Rsrc rr = rsrcBelow(rec, O.lo);
   const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairF));
-  const Rsrc rtabR = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoV));
+  const Rsrc rtabD = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoD));
   const uint32_t o16 = static_cast<uint32_t>(s) * 16u;
   const uint32_t pairBase = static_cast<uint32_t>(ei * 3) * static_cast<uint32_t>(tb.nr);
   const uint32_t* sh = O.sh[p];
   const float4* D = dl + p * kMaxLatOffsets;
   const float rc2 = static_cast<float>(G.rc2);
-  const int nk = O.n[p];
   Sum3<ACC> acc;
-  auto group = [&](auto UC, int k0) {
+  // the offsets the density pass found in range for some lane of this wavefront
+  uint64_t b0 = wmask[2 * waveId()], b1 = wmask[2 * waveId() + 1];
+  int left = __builtin_popcountll(b0) + __builtin_popcountll(b1);
+  auto group = [&](auto UC, const int* ks) {
     constexpr int V = decltype(UC)::value;
-    float4 nb[V];
-#pragma unroll
-    for (int u = 0; u < V; ++u) nb[u] = ldRec4(rr, o16, sh[k0 + u] * 16u);
-    bool in[V];
-    float d0[V], d1[V], d2[V], pp[V], ri[V];
-    float rec8[V][kPairRec], reci[V][kRhoRec];
+    float4 nb[V], d[V];
 #pragma unroll
     for (int u = 0; u < V; ++u) {
-      const float4 d = D[k0 + u];
-      d0[u] = (me.x - nb[u].x) - d.x;
-      d1[u] = (me.y - nb[u].y) - d.y;
-      d2[u] = (me.z - nb[u].z) - d.z;
-      const float r2 = d0[u] * d0[u] + d1[u] * d1[u] + d2[u] * d2[u];
-      in[u] = r2 < rc2;
-      if (in[u]) {
-        ri[u] = __builtin_amdgcn_rsqf(r2);
-        const int m = knotOf(r2 * ri[u], tb.rdr, tb.nr, false, pp[u]);
-        const int ej = recSpecies(nb[u].x);
-        const uint32_t off = (pairBase + static_cast<uint32_t>(ej * tb.nr + m)) *
-                             static_cast<uint32_t>(kPairRec * sizeof(float));
-        ldRec<float, kPairRec>(rtab, off, rec8[u]);
-        if (ej != ei) {  // rho_ei differs from rho_ej only for unlike pairs
-          const uint32_t offi = static_cast<uint32_t>(ei * tb.nr + m) * static_cast<uint32_t>(kRhoRec * sizeof(float));
-          ldRec<float, kRhoRec>(rtabR, offi, reci[u]);
-        } else {
-#pragma unroll
-          for (int q = 0; q < kRhoRec; q++) reci[u][q] = rec8[u][4 + q];
-        }
-      }
+      nb[u] = ldRec4(rr, o16, sh[ks[u]] * 16u);
+      d[u] = D[ks[u]];
     }
-#pragma unroll
-    for (int u = 0; u < V; ++u) {
-      if (in[u]) {
-        const float q = pp[u], rinv = ri[u];
-        const float* z = rec8[u];
-        const float* rj = rec8[u] + 4;
-        const float* rI = reci[u];
-        const float z2 = ((z[0] * q + z[1]) * q + z[2]) * q + z[3];
-        const float z2p = ((3.0f * z[0] * q + 2.0f * z[1]) * q + z[2]) * tb.rdr;
-        const float rhojp = ((3.0f * rj[0] * q + 2.0f * rj[1]) * q + rj[2]) * tb.rdr;
-        const float rhoip = ((3.0f * rI[0] * q + 2.0f * rI[1]) * q + rI[2]) * tb.rdr;
-        const float phi = z2 * rinv;
-        const float phip = z2p * rinv - phi * rinv;
-        const float psip = dfi * rhojp + nb[u].w * rhoip + phip;
-        acc.add(d0[u], d1[u], d2[u], -psip * rinv);
-      }
-    }
+    forceGroupM<ACC, V>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, acc);
   };
-  int k = 0;
-  for (; k + U <= nk; k += U) group(std::integral_constant<int, U>{}, k);
-  for (; k < nk; ++k) group(std::integral_constant<int, 1>{}, k);
-  if (ti & kNearInter) {  // forces from inter atoms on lattice neighbours (atom.cpp:397-422)
-    const int ne = static_cast<int>(iv.ctl->n_inter + iv.ctl->n_ghost);
-    const FullPlaneOffsets& full = *iv.full;
-    auto bucket = [&](int64_t q) {
-      forBucket(iv, q, ne, [&](int e) {
-        const int src = iv.e_src[e];
-        const double e0 = xi0 - iv.e_x[0][e], e1 = xi1 - iv.e_x[1][e], e2 = xi2 - iv.e_x[2][e];
-        const double r2 = e0 * e0 + e1 * e1 + e2 * e2;
-        if (r2 < G.rc2) acc.add(e0, e1, e2, pairForce(tb, ei, species(iv.type[src]), r2, dfi, iv.df32[src]));
-      });
-    };
-    bucket(s);
-    for (int q = 0; q < full.n[p]; q++) bucket(s + full.off[p][q]);
-  }
-  if (ti & kWide) {  // full-list offsets beyond the pruned list, near a far lattice atom
-    const FullPlaneOffsets& full = *iv.full;
-    for (int q = 0; q < full.nw[p]; q++) {
-      const int64_t j = s + full.woff[p][q];
-      const int8_t tj = type[j];
-      const double e0 = xi0 - x.c[0][j], e1 = xi1 - x.c[1][j], e2 = xi2 - x.c[2][j];
-      const double r2 = e0 * e0 + e1 * e1 + e2 * e2;
-      if (!isVacant(tj) && r2 < G.rc2) acc.add(e0, e1, e2, pairForce(tb, ei, species(tj), r2, dfi, rec[j].w));
+  for (; left > 0;) {
+    const int V = left >= U ? U : 1;
+    int ks[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (u < V) {
+        if (b0) {
+          ks[u] = __builtin_ctzll(b0);
+          b0 &= b0 - 1;
+        } else {
+          ks[u] = 64 + __builtin_ctzll(b1);
+          b1 &= b1 - 1;
+        }
+      } else {
+        ks[u] = 0;
+      }
     }
+    if (V == U) group(std::integral_constant<int, U>{}, ks);
+    else group(std::integral_constant<int, 1>{}, ks);
+    left -= V;
   }
-  double f0, f1, f2;
-  sum3Value<ACC>(acc, f0, f1, f2);
-  const StepDt sd = stepDt(vp);
-  const double c = sd.c[ei];
-  double v0 = v.c[0][s] + c * f0;
-  double v1 = v.c[1][s] + c * f1;
-  double v2 = v.c[2][s] + c * f2;
-  if (MODE != 1) {
-    f.c[0][s] = f0;
-    f.c[1][s] = f1;
-    f.c[2][s] = f2;
-    if (MODE == 0) {
-      v.c[0][s] = v0;
-      v.c[1][s] = v1;
-      v.c[2][s] = v2;
-    }
-    return;
-  }
-  v0 = v0 + c * f0;
-  v1 = v1 + c * f1;
-  v2 = v2 + c * f2;
-  const double y0 = xi0 + sd.dt * v0;
-  const double y1 = xi1 + sd.dt * v1;
-  const double y2 = xi2 + sd.dt * v2;
-  if (leftSite(G, p, cx, cy, cz, y0, y1, y2)) {
-    stageInter(st, ctl, refIndex(G, p, cx, cy, cz), s, id[s], ti, y0, y1, y2, v0, v1, v2);
-    v0 = v1 = v2 = 0.0;
-  }
-  v.c[0][s] = v0;
-  v.c[1][s] = v1;
-  v.c[2][s] = v2;
-  xn.c[0][s] = y0;
-  xn.c[1][s] = y1;
-  xn.c[2][s] = y2;
-  const float4 r = siteRec(G, p, cx, cy, cz, y0, y1, y2, ei);
-  recn[s] = r;
-  forEachImage(G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
-    xn.c[0][q] = y0 + sx * G.L0;
-    xn.c[1][q] = y1 + sy * G.L1;
-    xn.c[2][q] = y2 + sz * G.L2;
-    recn[q] = r;
-  });
+  forceTailM<ACC, MODE>(G, s, p, cx, cy, cz, ti, ei, dfi, acc, x, xn, type, rec, recn, v, f, id, vp, st, ctl, tb,
+                        iv);
 }
 
-constexpr int kUFloat = 4;
 constexpr int kUDouble = 2;
 
 }  // namespace
@@ -657,10 +481,22 @@ void launchRhoDf(const DevState& s, const LatOffsets& o, const Tables<double>& t
   const CXPtr x = cxp(s.x[s.cur]);
   if (!mixed)
     kRhoDf<double, 0, 4><<<grid, kBlock, 0, st>>>(s.G, o, t64, x, s.type, s.df64, rho, iv);
-  else if (acc == 0)
-    kRhoDfM<0, kUFloat><<<grid, kBlock, 0, st>>>(s.G, o, t32, x, s.type, s.rec[s.cur], s.dl, rho, iv);
-  else
-    kRhoDfM<1, kUFloat><<<grid, kBlock, 0, st>>>(s.G, o, t32, x, s.type, s.rec[s.cur], s.dl, rho, iv);
+  else {
+#define MISA_RHO_M(A, U) \
+  kRhoDfM<A, U><<<grid, kBlock, 0, st>>>(s.G, o, t32, x, s.type, s.rec[s.cur], s.dl, rho, s.wmask, iv)
+    const bool u8 = s.uRho == 8;
+    if (acc == 0) {
+      if (u8) MISA_RHO_M(0, 8);
+      else MISA_RHO_M(0, 4);
+    } else if (acc == 1) {
+      if (u8) MISA_RHO_M(1, 8);
+      else MISA_RHO_M(1, 4);
+    } else {
+      if (u8) MISA_RHO_M(2, 8);
+      else MISA_RHO_M(2, 4);
+    }
+#undef MISA_RHO_M
+  }
 }
 
 void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t64, const Tables<float>& t32,
@@ -673,9 +509,16 @@ void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t
 #define MISA_FORCE(M) \
   kForce<double, 0, M, kUDouble><<<grid, kBlock, 0, st>>>(s.G, o, t64, x, xn, s.type, s.df64, xp(s.v), xp(s.f), \
                                                          s.id, vp, sa, s.ctl, iv)
-#define MISA_FORCE_M(A, M)                                                                                       \
-  kForceM<A, M, kUFloat><<<grid, kBlock, 0, st>>>(s.G, o, t32, x, xn, s.type, s.rec[s.cur], s.rec[1 - s.cur], \
-                                                  s.dl, xp(s.v), xp(s.f), s.id, vp, sa, s.ctl, iv)
+#define MISA_FORCE_MU(A, M, U)                                                                              \
+  kForceM<A, M, U><<<grid, kBlock, 0, st>>>(s.G, o, t32, x, xn, s.type, s.rec[s.cur], s.rec[1 - s.cur], s.dl, \
+                                            s.wmask, xp(s.v), xp(s.f), s.id, vp, sa, s.ctl, iv)
+#define MISA_FORCE_M(A, M)            \
+  do {                                \
+    if (s.uForce == 2)                \
+      MISA_FORCE_MU(A, M, 2);         \
+    else                              \
+      MISA_FORCE_MU(A, M, 4);         \
+  } while (0)
   if (!mixed) {
     if (mode == 0) MISA_FORCE(0);
     else if (mode == 1) MISA_FORCE(1);
@@ -684,13 +527,18 @@ void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t
     if (mode == 0) MISA_FORCE_M(0, 0);
     else if (mode == 1) MISA_FORCE_M(0, 1);
     else MISA_FORCE_M(0, 2);
-  } else {
+  } else if (acc == 1) {
     if (mode == 0) MISA_FORCE_M(1, 0);
     else if (mode == 1) MISA_FORCE_M(1, 1);
     else MISA_FORCE_M(1, 2);
+  } else {
+    if (mode == 0) MISA_FORCE_M(2, 0);
+    else if (mode == 1) MISA_FORCE_M(2, 1);
+    else MISA_FORCE_M(2, 2);
   }
 #undef MISA_FORCE
 #undef MISA_FORCE_M
+#undef MISA_FORCE_MU
 }
 
 }  // namespace misa

@@ -552,8 +552,9 @@ __global__ __launch_bounds__(kIT) void kInterForce(Geom G, Tables<T> tb, InterVi
     in.f[0][i] = f0;
     in.f[1][i] = f1;
     in.f[2][i] = f2;
-    if (MODE == 0) {
-      const double c = stepDt(vp).c[ei];
+    if (MODE == 0) {  // secondstep, each product and sum rounded separately
+#pragma clang fp contract(off)
+      const double c = stepDt(vp, ei).c;
       in.v[0][i] += c * f0;
       in.v[1][i] += c * f1;
       in.v[2][i] += c * f2;
@@ -563,10 +564,11 @@ __global__ __launch_bounds__(kIT) void kInterForce(Geom G, Tables<T> tb, InterVi
 
 // NewtonMotion::firststep for the inter list (newton_motion.cpp:46-54).
 __global__ __launch_bounds__(kIT) void kInterFirst(InterDev in, const Ctl* ctl, VerletParams vp) {
+#pragma clang fp contract(off)  // the half kick and drift round as newton_motion.cpp does
   const int nl = static_cast<int>(ctl->n_inter);
   for (int i = blockIdx.x * kIT + threadIdx.x; i < nl; i += kIT * kIB) {
-    const StepDt sd = stepDt(vp);
-    const double c = sd.c[species(in.type[i])];
+    const StepDt sd = stepDt(vp, species(in.type[i]));
+    const double c = sd.c;
     for (int d = 0; d < 3; d++) {
       const double v = in.v[d][i] + c * in.f[d][i];
       in.v[d][i] = v;
@@ -747,11 +749,26 @@ __device__ __forceinline__ void blockMaxToCtl(unsigned long long m, Ctl* ctl) {
   }
 }
 
+// Grid-stride over the sub-box sites of both planes with a fixed number of
+// blocks, so that only a few hundred atomics meet at the one counter.
+constexpr int kVmaxBlocks = 512;
 __global__ __launch_bounds__(kBlock) void kVmaxSites(Geom G, CXPtr v, const int8_t* __restrict__ type, Ctl* ctl) {
-  int64_t s;
-  int p, cx, cy, cz;
   unsigned long long m = 0;
-  if (siteOfThread(G, s, p, cx, cy, cz) && !isVacant(type[s])) m = speed2Bits(v.c[0][s], v.c[1][s], v.c[2][s]);
+  const int64_t n = 2 * G.nsub;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+       i += static_cast<int64_t>(kVmaxBlocks) * kBlock) {
+    const int p = i >= G.nsub ? 1 : 0;
+    int64_t r = i - p * G.nsub;
+    const int cx = static_cast<int>(r % G.n0);
+    r /= G.n0;
+    const int cy = static_cast<int>(r % G.n1);
+    const int cz = static_cast<int>(r / G.n1);
+    const int64_t s = slotOf(G, p, cx, cy, cz);
+    if (!isVacant(type[s])) {
+      const unsigned long long b = speed2Bits(v.c[0][s], v.c[1][s], v.c[2][s]);
+      m = b > m ? b : m;
+    }
+  }
   blockMaxToCtl<kBlock>(m, ctl);
 }
 
@@ -776,7 +793,7 @@ __global__ void kDtUpdate(Ctl* ctl, double dt_max, double max_disp) {
 }  // namespace
 
 void launchAdaptiveDt(const DevState& s, double dt_max, double max_disp, hipStream_t st) {
-  kVmaxSites<<<siteGrid(s.G), kBlock, 0, st>>>(s.G, cxp(s.v), s.type, s.ctl);
+  kVmaxSites<<<kVmaxBlocks, kBlock, 0, st>>>(s.G, cxp(s.v), s.type, s.ctl);
   kVmaxInter<<<kIB, kIT, 0, st>>>(s.in, s.ctl);
   kDtUpdate<<<1, 1, 0, st>>>(s.ctl, dt_max, max_disp);
 }

@@ -39,14 +39,15 @@ using namespace site;
 __global__ __launch_bounds__(kBlock) void kFirst(Geom G, CXPtr x, XPtr xn, const int8_t* __restrict__ type,
                                                  XPtr v, CXPtr f, const uint64_t* __restrict__ id, VerletParams vp,
                                                  StageArgs st, Ctl* ctl, float4* __restrict__ recn) {
+#pragma clang fp contract(off)  // the half kick and drift round as newton_motion.cpp does
   int64_t s;
   int p, cx, cy, cz;
   if (!siteOfThread(G, s, p, cx, cy, cz)) return;
   const int8_t ti = type[s];
   double y0 = x.c[0][s], y1 = x.c[1][s], y2 = x.c[2][s];
   if (!isVacant(ti)) {
-    const StepDt sd = stepDt(vp);
-    const double c = sd.c[species(ti)];
+    const StepDt sd = stepDt(vp, species(ti));
+    const double c = sd.c;
     double v0 = v.c[0][s] + c * f.c[0][s];
     double v1 = v.c[1][s] + c * f.c[1][s];
     double v2 = v.c[2][s] + c * f.c[2][s];

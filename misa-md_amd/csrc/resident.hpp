@@ -68,6 +68,11 @@ struct DevState {
   // lattice vector of every pruned offset per centre parity
   float4* rec[2] = {nullptr, nullptr};
   float4* dl = nullptr;
+  // per wavefront of the site kernels: the pruned offsets within the cutoff
+  // for some lane, from the mixed density pass for the force pass (2 words)
+  uint64_t* wmask = nullptr;
+  // gathers in flight per wave in the mixed passes (chosen by misa_autotune)
+  int uRho = 4, uForce = 4;
   InterDev in;
   Ctl* ctl = nullptr;
   double* partials = nullptr;  // reduction scratch

@@ -101,6 +101,7 @@ __device__ __forceinline__ bool leftSite(const Geom& G, int p, int cx, int cy, i
 // distance test against it fails.  Periodic images carry the same record.
 __device__ __forceinline__ void sitePos(const Geom& G, int p, int cx, int cy, int cz, double& X, double& Y,
                                         double& Z) {
+#pragma clang fp contract(off)
   const double h = 0.5 * p;
   X = (cx + h) * G.a;
   Y = (cy + h) * G.a;
@@ -118,29 +119,30 @@ __device__ __forceinline__ int recSpecies(float ux) { return static_cast<int>(__
 // Record of the atom at x on site (p, cx, cy, cz) of species sp.
 __device__ __forceinline__ float4 siteRec(const Geom& G, int p, int cx, int cy, int cz, double x0, double x1,
                                           double x2, int sp) {
+#pragma clang fp contract(off)  // one rounding of the site position, one of the difference
   double X, Y, Z;
   sitePos(G, p, cx, cy, cz, X, Y, Z);
   return packRec(x0 - X, x1 - Y, x2 - Z, sp, 0.0f);
 }
 
-// Step length and half-kick factors of the current step: the fixed values,
-// or those of the adaptive step in device memory, computed as
+// Step length and half-kick factor of species t for the current step: the
+// fixed values, or those of the adaptive step in device memory, computed as
 // NewtonMotion::preComputeDtInv2m does (newton_motion.cpp:19-28).
 struct StepDt {
   double dt;
-  double c[3];
+  double c;
 };
-__device__ __forceinline__ StepDt stepDt(const VerletParams& vp) {
+__device__ __forceinline__ StepDt stepDt(const VerletParams& vp, int t) {
 #pragma clang fp contract(off)
   StepDt r;
   if (!vp.dtp) {
     r.dt = vp.dt;
-    for (int t = 0; t < 3; t++) r.c[t] = vp.c[t];
+    r.c = t == 0 ? vp.c[0] : (t == 1 ? vp.c[1] : vp.c[2]);
     return r;
   }
   r.dt = *vp.dtp;
   const double h = 0.5 * r.dt * kFtm2vDev;
-  for (int t = 0; t < 3; t++) r.c[t] = h / speciesMass(t);
+  r.c = h / speciesMass(t);
   return r;
 }
 
@@ -202,6 +204,14 @@ struct Sum1<1> {
   __device__ __forceinline__ double value() const { return static_cast<double>(s); }
 };
 
+// plain FP32 sums (ACC 2)
+template <>
+struct Sum1<2> {
+  float s = 0;
+  __device__ __forceinline__ void add(float v) { s += v; }
+  __device__ __forceinline__ double value() const { return static_cast<double>(s); }
+};
+
 template <int ACC>
 struct Sum3;
 template <>
@@ -225,10 +235,25 @@ struct Sum3<1> {
     c.add(static_cast<float>(d2) * fp);
   }
 };
+template <>
+struct Sum3<2> {
+  float f0 = 0, f1 = 0, f2 = 0;
+  __device__ __forceinline__ void add(float d0, float d1, float d2, float fp) {
+    f0 = fmaf(d0, fp, f0);
+    f1 = fmaf(d1, fp, f1);
+    f2 = fmaf(d2, fp, f2);
+  }
+};
 template <int ACC>
 __device__ __forceinline__ void sum3Value(const Sum3<ACC>& s, double& f0, double& f1, double& f2);
 template <>
 __device__ __forceinline__ void sum3Value<0>(const Sum3<0>& s, double& f0, double& f1, double& f2) {
+  f0 = s.f0;
+  f1 = s.f1;
+  f2 = s.f2;
+}
+template <>
+__device__ __forceinline__ void sum3Value<2>(const Sum3<2>& s, double& f0, double& f1, double& f2) {
   f0 = s.f0;
   f1 = s.f1;
   f2 = s.f2;

@@ -23,6 +23,7 @@ assert ATOM_DTYPE.itemsize == 104
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_MIXED_KAHAN = 17
+PREC_MIXED_F32SUM = 33
 
 A_FE = 2.855
 RC_FACTOR = 5.6 / 2.855
@@ -77,6 +78,9 @@ _SIGS = {
     "misa_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "misa_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
                                      ctypes.POINTER(ctypes.c_double)]),
+    "misa_set_launch_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "misa_get_launch_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
+                                             ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "misa_sim_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_double)]),
     "misa_compute_forces": (ctypes.c_int, [ctypes.c_void_p]),
@@ -233,6 +237,16 @@ class EamDevice:
         t, dt = ctypes.c_double(), ctypes.c_double()
         self._check(self.lib.misa_sim_time(self.h, ctypes.byref(t), ctypes.byref(dt)), "misa_sim_time")
         return t.value, dt.value
+
+    def set_launch_shape(self, ty, gathers_rho, gathers_force):
+        self._check(self.lib.misa_set_launch_shape(self.h, int(ty), int(gathers_rho), int(gathers_force)),
+                    "misa_set_launch_shape")
+
+    def launch_shape(self):
+        a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self._check(self.lib.misa_get_launch_shape(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+                    "misa_get_launch_shape")
+        return a.value, b.value, c.value
 
     def autotune(self, steps):
         ty, ms = ctypes.c_int32(), ctypes.c_double()

@@ -79,10 +79,12 @@ def test_autotune_leaves_the_trajectory_alone(setfl_fecuni):
     cells = (8, 8, 12)
     a, b = _box(setfl_fecuni, cells, 21, 300.0, (985, 10, 5), M.PREC_MIXED)
     ty, ms = a.autotune(2)
-    trials = len([t for t in range(1, min(32, cells[1]) + 1) if cells[1] % t == 0])
+    # mixed precision: every y-tile height with 4 | 8 density and 2 | 4 force gathers in flight
+    trials = 4 * len([t for t in range(1, min(32, cells[1]) + 1) if cells[1] % t == 0])
     b.step(2 * trials)
-    print("autotune: ty %d, %.3f ms/step over %d candidates" % (ty, ms, trials), flush=True)
-    assert cells[1] % ty == 0
+    print("autotune: ty %d, shape %s, %.3f ms/step over %d candidates" % (ty, a.launch_shape(), ms, trials),
+          flush=True)
+    assert cells[1] % ty == 0 and a.launch_shape()[0] == ty
     _same(a, b, "after autotune")
     a.step(4)
     b.step(4)
