@@ -794,10 +794,10 @@ StepPlan planOf(misa_handle* h) {
   return p;
 }
 
-void enqFirstAndDecide(misa_handle* h, const StepPlan& p, bool lattice) {
+void enqFirstAndDecide(misa_handle* h, const StepPlan& p, bool lattice, bool vmax_done = false) {
   {
     Timer t(h, 2);
-    if (p.adaptive) launchAdaptiveDt(h->s, h->dt_max, h->max_disp, h->stream);
+    if (p.adaptive) launchAdaptiveDt(h->s, h->dt_max, h->max_disp, !vmax_done, h->stream);
     if (lattice) launchFirst(h->s, p.mixed, p.vp, h->stream);
     launchInterFirst(h->s, p.vp, h->stream);
   }
@@ -810,8 +810,11 @@ void enqFirstAndDecide(misa_handle* h, const StepPlan& p, bool lattice) {
 
 void enqBody(misa_handle* h, const StepPlan& p, bool last) {
   const bool fuse = !last && !p.adaptive;
-  forcePass(h, fuse ? 1 : 0, 0, last, p.vp);
-  if (!last) enqFirstAndDecide(h, p, !fuse);
+  VerletParams vp = p.vp;
+  // adaptive: the secondstep pass reduces the lattice sites' max |v|^2 itself
+  if (p.adaptive && !last) vp.vmax = &h->s.ctl->vmax2;
+  forcePass(h, fuse ? 1 : 0, 0, last, vp);
+  if (!last) enqFirstAndDecide(h, p, !fuse, p.adaptive);
 }
 
 // Error flags of the step kernels; timings of the profiled launches.

@@ -150,7 +150,8 @@ template <int MODE>
 __device__ __forceinline__ void verletTail(const Geom& G, int64_t s, int p, int cx, int cy, int cz, int8_t ti,
                                            int ei, double xi0, double xi1, double xi2, double f0, double f1,
                                            double f2, XPtr xn, XPtr v, XPtr f, const uint64_t* __restrict__ id,
-                                           const VerletParams& vp, const StageArgs& st, Ctl* ctl) {
+                                           const VerletParams& vp, const StageArgs& st, Ctl* ctl,
+                                           unsigned long long& vb) {
 #pragma clang fp contract(off)
   const StepDt sd = stepDt(vp, ei);
   const double c = sd.c;
@@ -170,6 +171,7 @@ __device__ __forceinline__ void verletTail(const Geom& G, int64_t s, int p, int 
     v.c[0][s] = v0;
     v.c[1][s] = v1;
     v.c[2][s] = v2;
+    vb = speed2Bits(v0, v1, v2);
     return;
   }
   v0 = v0 + c * f0;
@@ -206,6 +208,8 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
                                                  const int8_t* __restrict__ type, const T* __restrict__ df,
                                                  XPtr v, XPtr f, const uint64_t* __restrict__ id,
                                                  VerletParams vp, StageArgs st, Ctl* ctl, InterView iv) {
+  unsigned long long vb = 0;  // |v|^2 bits after the secondstep (MODE 0)
+  [&] {
   int64_t s;
   int p, cx, cy, cz;
   if (!siteOfThread(G, s, p, cx, cy, cz)) return;
@@ -331,7 +335,9 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
   }
   double f0, f1, f2;
   sum3Value<ACC>(acc, f0, f1, f2);
-  verletTail<MODE>(G, s, p, cx, cy, cz, ti, ei, xi0, xi1, xi2, f0, f1, f2, xn, v, f, id, vp, st, ctl);
+  verletTail<MODE>(G, s, p, cx, cy, cz, ti, ei, xi0, xi1, xi2, f0, f1, f2, xn, v, f, id, vp, st, ctl, vb);
+  }();
+  if (MODE == 0 && vp.vmax) waveMaxTo(vb, vp.vmax);
 }
 
 // ---------------------------------------------------------------------------
@@ -347,7 +353,17 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
 // one lane of a wavefront, recorded by the density pass for the force pass,
 // which then skips the others: both passes see the same positions and round
 // the distances the same way, so the skipped offsets have no pair.
-__device__ __forceinline__ uint32_t waveId() { return blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); }
+// Index of the wavefront in the grid, in a scalar register (threadIdx.x >> 6
+// is the same in every lane of a wavefront).
+__device__ __forceinline__ uint32_t waveId() {
+  return blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+// A wavefront-uniform 64-bit value moved to scalar registers.
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
 
 template <int ACC, int U>
 __global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<float> tb, CXPtr x,
@@ -411,6 +427,8 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
                                                   const uint64_t* __restrict__ wmask, XPtr v, XPtr f,
                                                   const uint64_t* __restrict__ id, VerletParams vp, StageArgs st,
                                                   Ctl* ctl, InterView iv) {
+  unsigned long long vb = 0;  // |v|^2 bits after the secondstep (MODE 0)
+  [&] {
   int64_t s;
   int p, cx, cy, cz;
   if (!siteOfThread(G, s, p, cx, cy, cz)) return;
@@ -432,7 +450,8 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
   const float rc2 = static_cast<float>(G.rc2);
   Sum3<ACC> acc;
   // the offsets the density pass found in range for some lane of this wavefront
-  uint64_t b0 = wmask[2 * waveId()], b1 = wmask[2 * waveId() + 1];
+  const uint32_t wv = waveId();
+  uint64_t b0 = uniform64(wmask[2 * wv]), b1 = uniform64(wmask[2 * wv + 1]);
   int left = __builtin_popcountll(b0) + __builtin_popcountll(b1);
   auto group = [&](auto UC, const int* ks) {
     constexpr int V = decltype(UC)::value;
@@ -466,7 +485,9 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
     left -= V;
   }
   forceTailM<ACC, MODE>(G, s, p, cx, cy, cz, ti, ei, dfi, acc, x, xn, type, rec, recn, v, f, id, vp, st, ctl, tb,
-                        iv);
+                        iv, vb);
+  }();
+  if (MODE == 0 && vp.vmax) waveMaxTo(vb, vp.vmax);
 }
 
 constexpr int kUDouble = 2;

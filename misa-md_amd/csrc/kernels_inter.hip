@@ -724,14 +724,6 @@ __global__ __launch_bounds__(kIT) void kInterEnergy(Geom G, Tables<double> tb, I
 // ---------------------------------------------------------------------------
 // adaptive step length: dt = min(dt_max, max_disp / max|v|)
 // ---------------------------------------------------------------------------
-// |v|^2 with each product and sum rounded separately; non-negative doubles
-// order like their bit patterns, so an integer atomic max finds the largest.
-__device__ __forceinline__ unsigned long long speed2Bits(double a, double b, double c) {
-#pragma clang fp contract(off)
-  const double v2 = (a * a + b * b) + c * c;
-  return static_cast<unsigned long long>(__double_as_longlong(v2));
-}
-
 template <int N>
 __device__ __forceinline__ void blockMaxToCtl(unsigned long long m, Ctl* ctl) {
   __shared__ unsigned long long red[N / 64];
@@ -792,8 +784,8 @@ __global__ void kDtUpdate(Ctl* ctl, double dt_max, double max_disp) {
 
 }  // namespace
 
-void launchAdaptiveDt(const DevState& s, double dt_max, double max_disp, hipStream_t st) {
-  kVmaxSites<<<kVmaxBlocks, kBlock, 0, st>>>(s.G, cxp(s.v), s.type, s.ctl);
+void launchAdaptiveDt(const DevState& s, double dt_max, double max_disp, bool sites, hipStream_t st) {
+  if (sites) kVmaxSites<<<kVmaxBlocks, kBlock, 0, st>>>(s.G, cxp(s.v), s.type, s.ctl);
   kVmaxInter<<<kIB, kIT, 0, st>>>(s.in, s.ctl);
   kDtUpdate<<<1, 1, 0, st>>>(s.ctl, dt_max, max_disp);
 }

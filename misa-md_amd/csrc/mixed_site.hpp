@@ -186,7 +186,7 @@ __device__ __forceinline__ void forceTailM(const Geom& G, int64_t s, int p, int 
                                            float4* __restrict__ recn, XPtr v, XPtr f,
                                            const uint64_t* __restrict__ id, const VerletParams& vp,
                                            const StageArgs& st, Ctl* ctl, const Tables<float>& tb,
-                                           const InterView& iv) {
+                                           const InterView& iv, unsigned long long& vb) {
 #pragma clang fp contract(off)  // the half kicks and drift round as newton_motion.cpp does
   const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
   if (ti & (kNearInter | kWide)) {
@@ -229,6 +229,7 @@ __device__ __forceinline__ void forceTailM(const Geom& G, int64_t s, int p, int 
       v.c[0][s] = v0;
       v.c[1][s] = v1;
       v.c[2][s] = v2;
+      vb = speed2Bits(v0, v1, v2);
     }
     return;
   }

@@ -88,6 +88,9 @@ struct VerletParams {
   // adaptive time step: the step length lives in device memory (Ctl::dt) and
   // dt, c are derived from it in the kernels; null for a fixed step
   const double* dtp = nullptr;
+  // when set, the secondstep pass (MODE 0) also reduces max |v|^2 of the
+  // sites it updates into this word (the adaptive step's velocity maximum)
+  unsigned long long* vmax = nullptr;
 };
 
 // Host launchers (kernels_resident.hip).
@@ -108,7 +111,9 @@ int64_t launchEnergy(const DevState& s, const LatOffsets& o, const Tables<double
 void launchScaleV(const DevState& s, double factor, hipStream_t st);
 // Adaptive step: max |v|^2 over lattice and inter atoms into Ctl::vmax2, then
 // Ctl::dt = min(dt_max, max_disp / |v|max) and Ctl::t_sim += Ctl::dt.
-void launchAdaptiveDt(const DevState& s, double dt_max, double max_disp, hipStream_t st);
+// dt = min(dt_max, max_disp / max|v|); with sites false the lattice sites'
+// maximum is already in Ctl::vmax2 (reduced by the secondstep pass).
+void launchAdaptiveDt(const DevState& s, double dt_max, double max_disp, bool sites, hipStream_t st);
 // Inter-atom work (kernels_inter.hip).
 // decide mode 0: the step's ownership pass (append new inter atoms, vacancy
 // recombination, migration, ghost images, index, flags); mode 1: migration,

@@ -125,6 +125,26 @@ __device__ __forceinline__ float4 siteRec(const Geom& G, int p, int cx, int cy, 
   return packRec(x0 - X, x1 - Y, x2 - Z, sp, 0.0f);
 }
 
+// |v|^2 with each product and sum rounded separately; non-negative doubles
+// order like their bit patterns, so an integer maximum finds the largest.
+__device__ __forceinline__ unsigned long long speed2Bits(double a, double b, double c) {
+#pragma clang fp contract(off)
+  const double v2 = (a * a + b * b) + c * c;
+  return static_cast<unsigned long long>(__double_as_longlong(v2));
+}
+
+// Maximum of b over the 64 lanes of the wavefront into *dst; every lane of
+// the wavefront must reach the call.  The word only grows, so a wavefront
+// whose maximum does not exceed the value it reads skips the atomic.
+__device__ __forceinline__ void waveMaxTo(unsigned long long b, unsigned long long* dst) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long y = __shfl_xor(b, o, 64);
+    b = y > b ? y : b;
+  }
+  if ((threadIdx.x & 63) == 0 && b > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax(dst, b);
+}
+
 // Step length and half-kick factor of species t for the current step: the
 // fixed values, or those of the adaptive step in device memory, computed as
 // NewtonMotion::preComputeDtInv2m does (newton_motion.cpp:19-28).
