@@ -13,8 +13,13 @@ in HBM; the timed steps are the first steps of the cascade.
 Prints one JSON line: the FP32-mixed rate is `value`; the plain FP64 rate
 from the same equilibrated state, the mixed-vs-FP64 force error, the
 roofline of the dominant kernel (HIP events on the library's stream over the
-timed steps) and the single-core CPU oracle rate on a bounded sample sit
-beside it.
+timed steps), the NVE energy drift of both precisions over 10,000 steps of a
+2M-atom FeCuNi box, and the single-core CPU oracle rate on a bounded sample
+sit beside it.
+
+Mixed precision here is FP32 splines and pair terms with FP32 Kahan-
+compensated per-atom sums; positions, velocities, the Verlet update and all
+reductions stay FP64.
 """
 import argparse
 import json
@@ -44,6 +49,16 @@ STEP_BYTES = {"fp64": 138, "mixed": 130}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "kforce_mixed_fecuni_200c_traffic.txt")
 
 
+# FP32 splines/pair terms, FP32 + Kahan per-atom sums
+MIXED = M.PREC_MIXED_KAHAN
+MVV2E = 1.0364269e-4  # amu A^2/ps^2 -> eV (pre_define.h)
+
+
+def total_energy(th, natoms):
+    """Kinetic (mvv/2 converted to eV) plus potential energy, per atom."""
+    return (th["kinetic"] * MVV2E + th["e_potential"]) / natoms
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -53,7 +68,7 @@ PKA_DIR = (1.0, 3.0, 5.0)  # the recoil direction of the cascade configuration
 
 def equilibrate(dev, steps):
     """Untimed: fixed 1 fs steps with a rescale to 300 K every 50 steps."""
-    dev.set_precision(M.PREC_MIXED)
+    dev.set_precision(MIXED)
     dev.set_adaptive_dt(0.001, 0.0)
     dev.compute_forces()
     for _ in range(max(1, steps // 50)):
@@ -62,19 +77,25 @@ def equilibrate(dev, steps):
     return dev.download()
 
 
-def run_precision(dev, state, mode, steps, warmup, pka_energy, pka_site):
-    """From the equilibrated state: the recoil, warm-up steps, then `steps`
+def run_precision(dev, state, mode, steps, warmup, pka_energy, pka_site, pkas=None):
+    """From the equilibrated state: the recoil(s), warm-up steps, then `steps`
     timed steps; HIP events around every launch on the library's stream split
-    the timed region by kernel."""
+    the timed region by kernel.  `pkas` lists (site, direction) pairs that
+    recoil together (setv for each, then one collision step)."""
     dev.upload(*state)
     dev.set_precision(mode)
     dev.set_adaptive_dt(0.001, 0.0)
     dev.compute_forces()
     if pka_energy > 0:
-        dev.collision_step(pka_site, PKA_DIR, pka_energy)
+        pkas = pkas or [(pka_site, PKA_DIR)]
+        for lat, d in pkas[:-1]:
+            dev.setv(lat, d, pka_energy)
+        dev.collision_step(pkas[-1][0], pkas[-1][1], pka_energy)
         dev.set_adaptive_dt(0.001, 0.01)
     if warmup:
         dev.step(warmup)
+    natoms = int(np.prod(dev.cells)) * 2
+    e_start = total_energy(dev.thermo(), natoms)
     dev.set_profiling(True)
     t0 = time.perf_counter()
     dev.step(steps)  # returns after the device has finished
@@ -84,7 +105,9 @@ def run_precision(dev, state, mode, steps, warmup, pka_energy, pka_site):
     t_sim, dt_last = dev.sim_time()
     th = dev.thermo()
     return secs, tim, {"t_sim_ps": t_sim, "dt_last_ps": dt_last, "n_inter": th["n_inter"],
-                       "n_vacancy": th["n_vacancy"], "max_speed": th["max_speed"]}
+                       "n_vacancy": th["n_vacancy"], "max_speed": th["max_speed"],
+                       "e_total_timed_start_eV_per_atom": e_start,
+                       "e_total_timed_end_eV_per_atom": total_energy(th, natoms)}
 
 
 def traffic_bytes():
@@ -103,7 +126,7 @@ def force_error(dev):
     dev.set_precision(M.PREC_FP64)
     dev.compute_forces()
     a64, _ = dev.download()
-    dev.set_precision(M.PREC_MIXED)
+    dev.set_precision(MIXED)
     dev.compute_forces()
     a32, _ = dev.download()
     sel = a64["type"] >= 0
@@ -116,6 +139,134 @@ def force_error(dev):
     err = np.linalg.norm(f32 - f64, axis=1) / norm
     del atoms, sel, g
     return float(err.max()), float(np.sqrt(np.mean(err * err))), float(rms)
+
+
+def energy_drift(pot, cells, steps, every=500):
+    """NVE from the world builder's 300 K state (FeCuNi 985:10:5, dt 1 fs):
+    total energy per atom (FP64 kinetic + FP64 potential, whatever precision
+    the step runs in) every `every` steps, for FP64 and mixed steps."""
+    dev = M.EamDevice((cells, cells, cells))
+    dev.make_offsets()
+    dev.set_potential(pot)
+    dev.build_world(2020, 300.0, (985, 10, 5), 1)
+    dev.set_timestep(0.001)
+    state = dev.download()
+    natoms = 2 * cells ** 3
+    out = {"workload": "BCC FeCuNi %d^3 cells (%d atoms), Fe:Cu:Ni 985:10:5, 300 K start, NVE, dt 1 fs, %d steps, "
+                       "total energy sampled every %d steps" % (cells, natoms, steps, every)}
+    for name, mode in (("fp64", M.PREC_FP64), ("mixed", MIXED)):
+        dev.upload(*state)
+        dev.set_precision(mode)
+        dev.set_adaptive_dt(0.001, 0.0)
+        dev.compute_forces()
+        t, e = [0.0], []
+        th = dev.thermo()
+        e.append(total_energy(th, natoms))
+        for k in range(steps // every):
+            dev.step(every)
+            th = dev.thermo()
+            e.append(total_energy(th, natoms))
+            t.append((k + 1) * every * 0.001)
+        e = np.array(e)
+        slope = float(np.polyfit(np.array(t), e - e[0], 1)[0]) if len(e) > 2 else 0.0
+        out[name] = {"e0_eV_per_atom": float(e[0]), "drift_eV_per_atom": float(e[-1] - e[0]),
+                     "max_dev_eV_per_atom": float(np.abs(e - e[0]).max()),
+                     "slope_eV_per_atom_per_ps": slope, "temperature_end": th["temperature"]}
+    dev.close()
+    return out
+
+
+def batch_workload(pot, args):
+    """64 independent boxes stepped together through captured step graphs:
+    n ~ U{10..60} cells per side, T ~ U[300, 1200] K, Cu per mille ~ U{0..20}
+    (no Ni), seeds base + r, dt 1 fs, FP32-mixed, each box's launch shape
+    picked by the autotuner.  Returns the aggregate atom-steps/s line."""
+    rs = np.random.RandomState(args.batch_seed)  # MT19937
+    boxes = []
+    for r in range(args.batch_boxes):
+        n = int(rs.randint(10, 61))
+        t = float(rs.uniform(300.0, 1200.0))
+        c = int(rs.randint(0, 21))
+        boxes.append((n, t, c, args.batch_seed + r))
+    devs, natoms, shapes = [], 0, []
+    for n, t, c, seed in boxes:
+        d = M.EamDevice((n, n, n))
+        d.make_offsets()
+        d.set_potential(pot)
+        d.build_world(seed, t, (1000 - c, c, 0), seed)
+        d.set_timestep(0.001)
+        d.set_precision(MIXED)
+        d.compute_forces()
+        d.autotune(4)
+        shapes.append(list(d.launch_shape()))
+        devs.append(d)
+        natoms += 2 * n ** 3
+    log("batch of %d boxes, %d atoms" % (len(devs), natoms))
+    b = M.EamBatch(devs)
+    b.step(args.warmup)  # returns after the device has finished
+    t0 = time.perf_counter()
+    b.step(args.steps)
+    secs = time.perf_counter() - t0
+    temps = [d.thermo()["temperature"] for d in devs]
+    b.close()
+    for d in devs:
+        d.close()
+    return {
+        "metric": "atom-steps/s, batch of %d FeCuNi EAM boxes (FP32-mixed)" % len(boxes),
+        "value": natoms * args.steps / secs, "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": secs * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (mixed, FP32+Kahan sums)", "data": "synthetic",
+        "config": {"workload": "batch: %d BCC boxes, n ~ U{10..60} cells, T ~ U[300,1200] K, Cu ~ U{0..20} per "
+                               "mille, dt 1 fs, master seed %d" % (len(boxes), args.batch_seed),
+                   "atoms": natoms, "boxes": [list(x) for x in boxes], "launch_shapes": shapes,
+                   "parallelism": "single GPU"},
+        "temperatures_end": temps,
+    }
+
+
+def large_workload(pot, args):
+    """54M atoms (300^3 cells) FeCuNi at 300 K with 4 simultaneous 10 keV
+    recoils at seeded sites along seeded isotropic directions, adaptive step;
+    FP32-mixed and FP64 from the same state: rates, force error and energy
+    change over the run."""
+    n = args.cells
+    natoms = 2 * n ** 3
+    dev = M.EamDevice((n, n, n))
+    dev.make_offsets()
+    dev.set_potential(pot)
+    dev.build_world(2020, 300.0, (985, 10, 5), 1)
+    dev.set_timestep(0.001)
+    log("equilibration")
+    state = equilibrate(dev, args.equil)
+    rng = np.random.default_rng(args.batch_seed)
+    pkas = []
+    for _ in range(4):
+        lat = tuple(int(v) for v in rng.integers(0, n, size=3)) + (int(rng.integers(2)),)
+        u = rng.normal(size=3)
+        pkas.append((lat, tuple(float(x) for x in u / np.linalg.norm(u))))
+    out = {}
+    for name, mode in (("fp64", M.PREC_FP64), ("mixed", MIXED)):
+        log("%s run" % name)
+        secs, tim, cas = run_precision(dev, state, mode, args.steps, args.warmup, args.pka_energy, None, pkas)
+        out[name] = {"value": natoms * args.steps / secs, "ms_per_step": secs * 1e3 / args.steps,
+                     "force_ms": tim["force_ms"] / max(1, tim["launches"][1]),
+                     "rho_ms": tim["rho_df_ms"] / max(1, tim["launches"][0]), "cascade": cas}
+    del state
+    log("force error")
+    fmax, frms, f_scale = force_error(dev)
+    dev.close()
+    return {
+        "metric": "atom-steps/s, 54M-atom FeCuNi EAM with 4 x 10 keV recoils (FP32-mixed)",
+        "value": out["mixed"]["value"], "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": out["mixed"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (mixed, FP32+Kahan sums)", "data": "synthetic",
+        "config": {"workload": "BCC FeCuNi %d^3 cells (%d atoms), 985:10:5, 300 K (%d equilibration steps), "
+                               "4 PKAs of %.0f eV at %s, adaptive dt = min(1 fs, 0.01 A / max|v|)"
+                               % (n, natoms, args.equil, args.pka_energy, str(pkas)),
+                   "atoms": natoms, "parallelism": "single GPU"},
+        "fp64": out["fp64"], "mixed": out["mixed"], "mixed_over_fp64": out["mixed"]["value"] / out["fp64"]["value"],
+        "force_err_mixed_vs_fp64": {"max": fmax, "rms": frms, "rms_force_eV_per_A": f_scale},
+    }
 
 
 def cpu_baseline(setfl):
@@ -143,11 +294,25 @@ def main():
     ap.add_argument("--equil", type=int, default=100, help="untimed equilibration steps")
     ap.add_argument("--pka-energy", type=float, default=5000.0, help="eV; 0 times equilibrium steps of 1 fs")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--drift-steps", type=int, default=10000, help="NVE steps of the energy-drift leg; 0 skips it")
+    ap.add_argument("--drift-cells", type=int, default=100)
+    ap.add_argument("--workload", choices=("cascade", "batch", "large"), default="cascade",
+                    help="cascade: the 16M-atom cascade (default); batch: 64 boxes through captured graphs; "
+                         "large: 54M atoms with 4 recoils (use --cells 300 --pka-energy 10000)")
+    ap.add_argument("--batch-boxes", type=int, default=64)
+    ap.add_argument("--batch-seed", type=int, default=20240)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs the EAM step on one GPU only; --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.workload != "cascade":
+        with tempfile.TemporaryDirectory() as tmp:
+            pot = os.path.join(tmp, "fecuni.eam.alloy")
+            setfl_gen.write_setfl(pot, ("Fe", "Cu", "Ni"))
+            res = batch_workload(pot, args) if args.workload == "batch" else large_workload(pot, args)
+        print(json.dumps(res))
+        return
     n = args.cells
     natoms = 2 * n ** 3
     with tempfile.TemporaryDirectory() as tmp:
@@ -165,12 +330,16 @@ def main():
         log("fp64 run")
         s64, t64, c64 = run_precision(dev, state, M.PREC_FP64, args.steps, args.warmup, args.pka_energy, site)
         log("mixed run")
-        s32, t32, c32 = run_precision(dev, state, M.PREC_MIXED, args.steps, args.warmup, args.pka_energy, site)
+        s32, t32, c32 = run_precision(dev, state, MIXED, args.steps, args.warmup, args.pka_energy, site)
         del state
         log("force error")
         fmax, frms, f_scale = force_error(dev)
         th = dev.thermo()
         dev.close()
+        drift = None
+        if args.drift_steps > 0:
+            log("energy drift")
+            drift = energy_drift(pot, args.drift_cells, args.drift_steps)
         cpu = None if args.no_cpu else cpu_baseline(pot)
 
     rate32 = natoms * args.steps / s32
@@ -203,7 +372,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (mixed: f32 splines/pair terms, f64 positions/Verlet/sums)",
+        "dtype": "f32 (mixed: f32 splines/pair terms, f32+Kahan per-atom sums, f64 positions/Verlet/reductions)",
         "data": "synthetic",
         "config": {"workload": ("collision cascade: BCC FeCuNi %d^3 cells (%d atoms), Fe:Cu:Ni 985:10:5, "
                                 "equilibrated at 300 K (%d steps of 1 fs), PKA %.0f eV at %s along [1 3 5], "
@@ -219,6 +388,7 @@ def main():
         "kernels": kernels,
         "mixed_over_fp64": rate32 / rate64,
         "force_err_mixed_vs_fp64": {"max": fmax, "rms": frms, "rms_force_eV_per_A": f_scale},
+        "energy_drift": drift,
         "roofline": {"bound": "hbm", "kernel": "kForceM (force + Verlet pass, mixed)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_bytes(), "algorithmic_bytes_per_launch": natoms * FORCE_BYTES["mixed"],
