@@ -89,3 +89,21 @@ def test_autotune_leaves_the_trajectory_alone(setfl_fecuni):
     a.step(4)
     b.step(4)
     _same(a, b, "after autotune + 4 steps")
+
+
+@pytest.mark.gpu
+def test_launch_shapes_give_one_trajectory(setfl_fecuni):
+    """Every launch shape of the mixed passes (y-tile, gathers in flight)
+    computes every site the same way: 30 steps of an alloy box at 900 K
+    agree bit for bit, and so do their forces."""
+    cells = (8, 8, 12)
+    ref, _ = _box(setfl_fecuni, cells, 33, 900.0, (900, 70, 30), M.PREC_MIXED_KAHAN)
+    ref.set_launch_shape(8, 4, 4)
+    ref.step(30)
+    for shape in [(1, 8, 2), (4, 4, 2), (2, 8, 4)]:
+        dev, _ = _box(setfl_fecuni, cells, 33, 900.0, (900, 70, 30), M.PREC_MIXED_KAHAN)
+        dev.set_launch_shape(*shape)
+        assert dev.launch_shape() == shape
+        dev.step(30)
+        _same(ref, dev, "shape %s" % (shape,))
+        dev.close()
