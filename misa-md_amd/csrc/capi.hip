@@ -452,7 +452,8 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
   const int64_t oRhoV = 3 * sF + 9 * sR;            // [3][nr][4] density value cubics
   const int64_t oPair = oRhoV + 3 * nr * kRhoRec;  // [9][nr][8] force records per ordered pair
   const int64_t oRhoD = oPair + 9 * nr * kPairRec;  // [3][nr][4] density slope cubics (FP32 force pass)
-  const int64_t total = oRhoD + 3 * nr * kRhoRec;
+  const int64_t oPairZ = oRhoD + 3 * nr * kRhoRec;  // [9][nr][4] pair value cubics alone (FP32 force pass)
+  const int64_t total = oPairZ + 9 * nr * kRhoRec;
   std::vector<double> h64(static_cast<size_t>(total), 0.0);
   auto put = [&](int64_t base, const std::vector<double>& c7, int n) {
     for (int m = 0; m < n; m++) {
@@ -494,9 +495,11 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
       const std::vector<double>& Z = h->cZ[ea * (ea + 1) / 2 + eb];
       for (int64_t m = 0; m < nr; m++) {
         double* r = &h64[oPair + ((a * 3 + b2) * nr + m) * kPairRec];
+        double* z = &h64[oPairZ + ((a * 3 + b2) * nr + m) * kRhoRec];
         for (int q = 0; q < 4; q++) {
           r[q] = Z[m * 7 + 3 + q];
           r[4 + q] = Rb[m * 7 + 3 + q];
+          z[q] = Z[m * 7 + 3 + q];
         }
       }
     }
@@ -542,6 +545,7 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
   a64.rhoV = h->tab64 + oRhoV;
   a64.pairF = h->tab64 + oPair;
   a64.rhoD = h->tab64 + oRhoD;
+  a64.pairZ = h->tab64 + oPairZ;
   Tables<float>& a32 = h->t32;
   a32.F = h->tab32;
   a32.rho = h->tab32 + 3 * sF;
@@ -556,6 +560,7 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
   a32.rhoV = h->tab32 + oRhoV;
   a32.pairF = h->tab32 + oPair;
   a32.rhoD = h->tab32 + oRhoD;
+  a32.pairZ = h->tab32 + oPairZ;
   h->have_pot = true;
   return MISA_OK;
 }

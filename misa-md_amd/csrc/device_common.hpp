@@ -112,6 +112,9 @@ struct Tables {
   // knot coordinate already divided by the knot spacing (3 c3/dr, 2 c4/dr, c5/dr)
   const T* pairF;
   const T* rhoD;   // [3][nr][4]: slope cubic of rho_e as in pairF (FP32 force pass)
+  // [9][nr][4]: value cubic of (r phi)_eiej alone, 16 B a knot, for the
+  // mixed force pass's gathers (eight knots to a 128-byte cache line)
+  const T* pairZ;
 };
 constexpr int kPairRec = 8;
 constexpr int kRhoRec = 4;

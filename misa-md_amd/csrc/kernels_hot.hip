@@ -441,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
   const float4 me = rec[s];
   const float dfi = me.w;
   const Rsrc rr = rsrcBelow(rec, O.lo);
-  const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairF));
+  const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairZ));
   const Rsrc rtabD = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoD));
   const uint32_t o16 = static_cast<uint32_t>(s) * 16u;
   const uint32_t pairBase = static_cast<uint32_t>(ei * 3) * static_cast<uint32_t>(tb.nr);

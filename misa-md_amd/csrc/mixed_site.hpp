@@ -55,9 +55,11 @@ __device__ __forceinline__ void rhoGroupM(const float4& me, const float4 (&nb)[V
 }
 
 // f_i += sum over the group of (x_i - x_j) fpair_ij (pot eam::toForce): the
-// record of the pair (the value cubic of r*phi, the slope cubic of the
-// neighbour's density) and, for unlike pairs, the slope cubic of the centre's
-// density.  A slope cubic is already divided by the knot spacing.
+// value cubic of r*phi of the pair, the slope cubic of the neighbour's
+// density and, for unlike pairs, the slope cubic of the centre's density,
+// each from a table of 16-byte knots (the gathers of a wavefront spread over
+// a range of knots, so the narrower the knot, the fewer cache lines a gather
+// touches).  A slope cubic is already divided by the knot spacing.
 template <int ACC, int V>
 __device__ __forceinline__ void forceGroupM(const float4& me, float dfi, int ei, uint32_t pairBase,
                                             const float4 (&nb)[V], const float4 (&d)[V], float rc2, Rsrc rtab,
@@ -75,9 +77,9 @@ __device__ __forceinline__ void forceGroupM(const float4& me, float dfi, int ei,
       ri[u] = __builtin_amdgcn_rsqf(r2);
       const int m = knotM(r2 * ri[u], tb.rdr, tb.nr, pp[u]);
       const uint32_t ej = static_cast<uint32_t>(recSpecies(nb[u].x));
-      const uint32_t off = (pairBase + __umul24(ej, static_cast<uint32_t>(tb.nr)) + static_cast<uint32_t>(m)) * 32u;
-      zc[u] = ldRec4(rtab, off, 0);
-      sj[u] = ld3(rtab, off + 16u);
+      const uint32_t rowj = __umul24(ej, static_cast<uint32_t>(tb.nr)) + static_cast<uint32_t>(m);
+      zc[u] = ldRec4(rtab, (pairBase + rowj) * 16u, 0);
+      sj[u] = ld3(rtabD, rowj * 16u);
       like[u] = ej == static_cast<uint32_t>(ei);
       si[u] = f32x3{0.0f, 0.0f, 0.0f};
       if (!like[u])  // rho_ei differs from rho_ej only for unlike pairs
