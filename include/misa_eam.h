@@ -205,6 +205,30 @@ int misa_autotune(misa_handle* h, int64_t steps, int32_t* best_ty, double* best_
 int misa_set_launch_shape(misa_handle* h, int32_t ty, int32_t gathers_rho, int32_t gathers_force);
 int misa_get_launch_shape(const misa_handle* h, int32_t* ty, int32_t* gathers_rho, int32_t* gathers_force);
 
+/* ---- frontend I/O: atom dumps (frontend/io), one rank ----
+ *
+ * Copy mode (AtomDump, frontend/io/atom_dump.cpp:44-101, with
+ * BufferedFileWriter, buffered_io.cpp:17-49): 72-byte AtomInfoDump records
+ * (atom_info_dump.h:14-22: id, step, type, inter_type 0, x, v) after a
+ * 128-byte file header (zeros) and a 128-byte rank header whose first 8
+ * bytes hold the record count, in the layout tools/dump2text/out_convert.cpp
+ * reads.  A frame is the inter atoms in list order, then the occupied
+ * sub-box sites of the ghost-extended AtomElement array in z, y, doubled-x
+ * order (atom_dump.cpp:66-84).  Frames append; close writes the count
+ * (AtomDump::writeDumpHeader). */
+typedef struct misa_dump misa_dump;
+int misa_dump_open(const char* path, misa_dump** out);
+int misa_dump_frame(misa_dump* d, const misa_geom* g, const void* atoms, const void* inter, int64_t n_inter,
+                    uint64_t time_step);
+/* The same for a handle's resident state (downloaded first). */
+int misa_dump_frame_state(misa_dump* d, misa_handle* h, uint64_t time_step);
+int misa_dump_close(misa_dump* d, int64_t* records);
+/* Debug mode (OutputDump::dump, output_dump.cpp:40-70): "print atoms", then
+ * "id x y z" of every occupied sub-box site, "print inter", then the inter
+ * atoms; numbers as std::ostream prints them by default (%g). */
+int misa_dump_text(const char* path, const misa_geom* g, const void* atoms, const void* inter, int64_t n_inter);
+int misa_dump_text_state(misa_handle* h, const char* path);
+
 /* ---- host-only set-up: no device access, usable without a GPU ---- */
 
 /* NeighbourIndex::make for geometry `g` (which = 0..3 as misa_get_offsets;

@@ -669,6 +669,39 @@ int misa_download_state(misa_handle* h, void* atoms, void* inter, int64_t inter_
   return MISA_OK;
 }
 
+// The handle's state, downloaded for the frontend's dumps.
+static int downloadForDump(misa_handle* h, std::vector<AtomElement>& atoms, std::vector<AtomElement>& inter,
+                           int64_t& n_inter, misa_geom& g) {
+  atoms.resize(static_cast<size_t>(h->box.slots()));
+  inter.resize(static_cast<size_t>(std::max<int64_t>(h->s.in.cap, 1)));
+  const int rc = misa_download_state(h, atoms.data(), inter.data(), h->s.in.cap, &n_inter);
+  if (rc != MISA_OK) return rc;
+  for (int k = 0; k < 3; k++) g.phase_space[k] = h->box.n[k];
+  g.lattice_const = h->box.a;
+  g.cutoff_radius_factor = h->box.f;
+  return MISA_OK;
+}
+
+int misa_dump_frame_state(misa_dump* d, misa_handle* h, uint64_t time_step) {
+  if (!d || !h) return MISA_E_ARG;
+  std::vector<AtomElement> atoms, inter;
+  int64_t n = 0;
+  misa_geom g;
+  const int rc = downloadForDump(h, atoms, inter, n, g);
+  if (rc != MISA_OK) return rc;
+  return misa_dump_frame(d, &g, atoms.data(), inter.data(), n, time_step);
+}
+
+int misa_dump_text_state(misa_handle* h, const char* path) {
+  if (!h || !path) return MISA_E_ARG;
+  std::vector<AtomElement> atoms, inter;
+  int64_t n = 0;
+  misa_geom g;
+  const int rc = downloadForDump(h, atoms, inter, n, g);
+  if (rc != MISA_OK) return rc;
+  return misa_dump_text(path, &g, atoms.data(), inter.data(), n);
+}
+
 int misa_build_world(misa_handle* h, uint32_t seed, double tset, const int32_t ratio[3], uint32_t alloy_seed) {
   if (!h || !ratio) return MISA_E_ARG;
   if (ratio[0] < 0 || ratio[1] < 0 || ratio[2] < 0 || ratio[0] + ratio[1] + ratio[2] <= 0)

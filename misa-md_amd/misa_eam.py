@@ -100,6 +100,14 @@ _SIGS = {
                                                  ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "misa_host_build_world": (ctypes.c_int, [ctypes.POINTER(MisaGeom), ctypes.c_uint32, ctypes.c_double,
                                              ctypes.POINTER(ctypes.c_int32), ctypes.c_uint32, ctypes.c_void_p]),
+    "misa_dump_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "misa_dump_frame": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MisaGeom), ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_int64, ctypes.c_uint64]),
+    "misa_dump_frame_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]),
+    "misa_dump_close": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "misa_dump_text": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(MisaGeom), ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_int64]),
+    "misa_dump_text_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
 }
 
 
@@ -374,3 +382,97 @@ def sub_box_view(atoms, dev):
     g = dev.ghost
     a = atoms.reshape(dev.dims[2], dev.dims[1], dev.dims[0])
     return a[g:g + dev.cells[2], g:g + dev.cells[1], 2 * g:2 * g + 2 * dev.cells[0]]
+
+
+# ---- atom dumps of the reference frontend (frontend/io) ----
+
+# AtomInfoDump (frontend/io/atom_info_dump.h:14-22) as written to a dump file
+DUMP_DTYPE = np.dtype([
+    ("id", "<u8"), ("step", "<u8"), ("type", "<i4"), ("inter_type", "<i2"), ("_pad", "<i2"),
+    ("x", "<f8", (3,)), ("v", "<f8", (3,)),
+])
+assert DUMP_DTYPE.itemsize == 72
+DUMP_HEADER = 128        # file header (the storage library's; zeros here)
+DUMP_RANK_HEADER = 128   # per rank: record count in the first 8 bytes
+DUMP_BLOCK = 1024        # records per block; blocks of the ranks interleave
+
+
+def _inter_arg(inter):
+    if inter is None or len(inter) == 0:
+        return None, 0
+    inter = np.ascontiguousarray(inter, dtype=ATOM_DTYPE)
+    return inter, len(inter)
+
+
+class AtomDumpFile:
+    """Copy-mode atom dump of one rank (AtomDump + BufferedFileWriter): frames
+    of the inter atoms in list order, then the occupied sub-box sites in z, y,
+    doubled-x order; close() writes the record count and returns it."""
+
+    def __init__(self, path):
+        self.lib = load()
+        d = ctypes.c_void_p()
+        if self.lib.misa_dump_open(str(path).encode(), ctypes.byref(d)) != 0:
+            raise MisaError("misa_dump_open failed for %s" % path)
+        self.d = d
+
+    def frame(self, atoms, inter, cells, step, lattice_const=A_FE, rc_factor=RC_FACTOR):
+        g = geom(cells, lattice_const, rc_factor)
+        atoms = np.ascontiguousarray(atoms, dtype=ATOM_DTYPE)
+        inter, n = _inter_arg(inter)
+        rc = self.lib.misa_dump_frame(self.d, ctypes.byref(g), _ptr(atoms), _ptr(inter) if n else None, n,
+                                      int(step))
+        if rc != 0:
+            raise MisaError("misa_dump_frame (%d)" % rc)
+
+    def frame_state(self, dev, step):
+        """A frame of a device handle's resident state."""
+        rc = self.lib.misa_dump_frame_state(self.d, dev.h, int(step))
+        if rc != 0:
+            raise MisaError("misa_dump_frame_state (%d)" % rc)
+
+    def close(self):
+        n = ctypes.c_int64()
+        rc = self.lib.misa_dump_close(self.d, ctypes.byref(n))
+        self.d = None
+        if rc != 0:
+            raise MisaError("misa_dump_close (%d)" % rc)
+        return n.value
+
+
+def dump_text(path, atoms, inter, cells, lattice_const=A_FE, rc_factor=RC_FACTOR):
+    """Debug-mode text dump (OutputDump::dump)."""
+    lib = load()
+    g = geom(cells, lattice_const, rc_factor)
+    atoms = np.ascontiguousarray(atoms, dtype=ATOM_DTYPE)
+    inter, n = _inter_arg(inter)
+    if lib.misa_dump_text(str(path).encode(), ctypes.byref(g), _ptr(atoms), _ptr(inter) if n else None, n) != 0:
+        raise MisaError("misa_dump_text")
+
+
+def dump_text_state(dev, path):
+    if dev.lib.misa_dump_text_state(dev.h, str(path).encode()) != 0:
+        raise MisaError("misa_dump_text_state")
+
+
+def read_dump(path, n_ranks=1):
+    """Records of a copy-mode dump, rank by rank, located the way
+    tools/dump2text/out_convert.cpp locates them: rank r's count at
+    DUMP_HEADER + DUMP_RANK_HEADER * r, its k-th block of DUMP_BLOCK records
+    at DUMP_HEADER + DUMP_RANK_HEADER * n_ranks + (n_ranks * k + r) blocks."""
+    raw = np.fromfile(str(path), dtype=np.uint8)
+    out = []
+    base = DUMP_HEADER + DUMP_RANK_HEADER * n_ranks
+    bb = DUMP_BLOCK * DUMP_DTYPE.itemsize
+    for r in range(n_ranks):
+        h = DUMP_HEADER + DUMP_RANK_HEADER * r
+        count = int(raw[h:h + 8].view("<u8")[0])
+        parts, left, k = [], count, 0
+        while left > 0:
+            take = min(left, DUMP_BLOCK)
+            o = base + (n_ranks * k + r) * bb
+            parts.append(raw[o:o + take * DUMP_DTYPE.itemsize].view(DUMP_DTYPE))
+            left -= take
+            k += 1
+        out.append(np.concatenate(parts) if parts else np.zeros(0, DUMP_DTYPE))
+    return out[0] if n_ranks == 1 else out
