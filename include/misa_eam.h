@@ -229,6 +229,70 @@ int misa_dump_close(misa_dump* d, int64_t* records);
 int misa_dump_text(const char* path, const misa_geom* g, const void* atoms, const void* inter, int64_t n_inter);
 int misa_dump_text_state(misa_handle* h, const char* path);
 
+/* ---- frontend: YAML configuration and the stage machine ----
+ *
+ * ConfigValues (frontend/config_values.h:29-109) as parsed by ConfigParser
+ * (frontend/config_parser.cpp:46-343) from the reference's YAML format
+ * (example/config.yaml), with the same defaults and error messages; and
+ * MDSimulation's stage machine (frontend/md_simulation.cpp:40-115) over the
+ * resident step: stage step lengths, recoils (when an origin dump path is
+ * set, as in the reference), rescales, atom dumps and thermo lines, with the
+ * steps between two such events in one misa_step call. */
+#define MISA_MAX_STAGES 64
+#define MISA_PATH_MAX 512
+#define MISA_DUMP_DEBUG 0 /* OutputMode::DEBUG: text dumps dump_<rank>_<step>.atom */
+#define MISA_DUMP_COPY 1  /* OutputMode::COPY: the binary copy-mode dump */
+typedef struct misa_stage {
+  uint64_t steps;
+  double step_length;      /* 0: the default step length */
+  int32_t collision_set;   /* set_v */
+  int32_t collision_lat[4];
+  uint64_t collision_step; /* relative to the stage; the recoil precedes stage step collision_step - 1 */
+  double pka_energy;
+  double direction[3];
+  int32_t rescale_set;
+  int32_t pad_;
+  double rescale_t;
+  uint64_t rescale_every;
+} misa_stage;
+typedef struct misa_config {
+  int64_t phase_space[3];
+  double cutoff_radius_factor;
+  double lattice_const;
+  uint64_t time_steps; /* sum of the stages' steps */
+  double time_step_length;
+  int32_t create_phase;
+  int32_t create_seed;
+  double create_t_set;
+  char read_phase_filename[MISA_PATH_MAX];
+  int32_t alloy_create_seed;
+  int32_t alloy_ratio[3];
+  char potential_file_type[32];
+  char potential_filename[MISA_PATH_MAX];
+  int32_t atoms_dump_mode;
+  int32_t out_by_frame;
+  uint64_t atoms_dump_interval;
+  char atoms_dump_file_path[MISA_PATH_MAX];
+  char origin_dump_path[MISA_PATH_MAX];
+  uint64_t thermo_interval;
+  int32_t logs_mode; /* 0 console, 1 file */
+  int32_t n_stages;
+  char logs_filename[MISA_PATH_MAX];
+  misa_stage stages[MISA_MAX_STAGES];
+} misa_config;
+typedef struct misa_md_report {
+  uint64_t steps;
+  double phy_time;
+  int64_t stages_run, collisions, rescales, dump_frames, dump_records, thermo_lines;
+  misa_thermo thermo; /* at the end */
+} misa_md_report;
+/* Parse a configuration file (or text); `err` receives the message. */
+int misa_config_parse(const char* path, misa_config* out, char* err, int64_t err_cap);
+int misa_config_parse_text(const char* text, misa_config* out, char* err, int64_t err_cap);
+/* Build the world of `c` (creation mode, setfl potential) on `device`, run
+ * every stage in `precision` (MISA_PREC_*), dump and log as configured. */
+int misa_md_simulate(const misa_config* c, int device, int32_t precision, misa_md_report* report);
+
 /* ---- host-only set-up: no device access, usable without a GPU ---- */
 
 /* NeighbourIndex::make for geometry `g` (which = 0..3 as misa_get_offsets;

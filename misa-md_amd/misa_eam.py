@@ -46,6 +46,44 @@ class MisaThermo(ctypes.Structure):
         return d
 
 
+MAX_STAGES = 64
+PATH_MAX = 512
+DUMP_DEBUG = 0
+DUMP_COPY = 1
+
+
+class MisaMdReport(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_uint64), ("phy_time", ctypes.c_double), ("stages_run", ctypes.c_int64),
+                ("collisions", ctypes.c_int64), ("rescales", ctypes.c_int64), ("dump_frames", ctypes.c_int64),
+                ("dump_records", ctypes.c_int64), ("thermo_lines", ctypes.c_int64), ("thermo", MisaThermo)]
+
+
+class MisaStage(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_uint64), ("step_length", ctypes.c_double), ("collision_set", ctypes.c_int32),
+                ("collision_lat", ctypes.c_int32 * 4), ("collision_step", ctypes.c_uint64),
+                ("pka_energy", ctypes.c_double), ("direction", ctypes.c_double * 3),
+                ("rescale_set", ctypes.c_int32), ("pad_", ctypes.c_int32), ("rescale_t", ctypes.c_double),
+                ("rescale_every", ctypes.c_uint64)]
+
+
+class MisaConfig(ctypes.Structure):
+    """ConfigValues (frontend/config_values.h) as the C-ABI holds it."""
+    _fields_ = [("phase_space", ctypes.c_int64 * 3), ("cutoff_radius_factor", ctypes.c_double),
+                ("lattice_const", ctypes.c_double), ("time_steps", ctypes.c_uint64),
+                ("time_step_length", ctypes.c_double), ("create_phase", ctypes.c_int32),
+                ("create_seed", ctypes.c_int32), ("create_t_set", ctypes.c_double),
+                ("read_phase_filename", ctypes.c_char * PATH_MAX), ("alloy_create_seed", ctypes.c_int32),
+                ("alloy_ratio", ctypes.c_int32 * 3), ("potential_file_type", ctypes.c_char * 32),
+                ("potential_filename", ctypes.c_char * PATH_MAX), ("atoms_dump_mode", ctypes.c_int32),
+                ("out_by_frame", ctypes.c_int32), ("atoms_dump_interval", ctypes.c_uint64),
+                ("atoms_dump_file_path", ctypes.c_char * PATH_MAX), ("origin_dump_path", ctypes.c_char * PATH_MAX),
+                ("thermo_interval", ctypes.c_uint64), ("logs_mode", ctypes.c_int32), ("n_stages", ctypes.c_int32),
+                ("logs_filename", ctypes.c_char * PATH_MAX), ("stages", MisaStage * MAX_STAGES)]
+
+    def stage_list(self):
+        return [self.stages[i] for i in range(self.n_stages)]
+
+
 _lib = None
 
 _SIGS = {
@@ -108,6 +146,9 @@ _SIGS = {
     "misa_dump_text": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(MisaGeom), ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_int64]),
     "misa_dump_text_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "misa_config_parse": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
+    "misa_config_parse_text": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
+    "misa_md_simulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int32, ctypes.c_void_p]),
 }
 
 
@@ -476,3 +517,31 @@ def read_dump(path, n_ranks=1):
             k += 1
         out.append(np.concatenate(parts) if parts else np.zeros(0, DUMP_DTYPE))
     return out[0] if n_ranks == 1 else out
+
+
+# ---- frontend: YAML configuration and the stage machine ----
+
+def parse_config(path=None, text=None):
+    """ConfigValues from a configuration file (or text); raises MisaError
+    with the parser's message."""
+    lib = load()
+    cfg = MisaConfig()
+    err = ctypes.create_string_buffer(512)
+    if text is not None:
+        rc = lib.misa_config_parse_text(text.encode(), ctypes.byref(cfg), err, 512)
+    else:
+        rc = lib.misa_config_parse(str(path).encode(), ctypes.byref(cfg), err, 512)
+    if rc != 0:
+        raise MisaError("configuration: %s" % err.value.decode())
+    return cfg
+
+
+def md_simulate(cfg, device=0, precision=PREC_FP64):
+    """Run every stage of `cfg` on the device (MDSimulation); returns the
+    report (steps, physical time, events, final thermo)."""
+    lib = load()
+    rep = MisaMdReport()
+    rc = lib.misa_md_simulate(ctypes.byref(cfg), device, precision, ctypes.byref(rep))
+    if rc != 0:
+        raise MisaError("misa_md_simulate (%d)" % rc)
+    return rep
