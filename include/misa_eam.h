@@ -205,6 +205,46 @@ int misa_autotune(misa_handle* h, int64_t steps, int32_t* best_ty, double* best_
 int misa_set_launch_shape(misa_handle* h, int32_t ty, int32_t gathers_rho, int32_t gathers_force);
 int misa_get_launch_shape(const misa_handle* h, int32_t* ty, int32_t* gathers_rho, int32_t* gathers_force);
 
+/* ---- host mirror of libcomm's domain at one rank ----
+ *
+ * comm::BccDomain (built as simulation::createDomain does,
+ * simulation.cpp:28-50, ghost size ceil(f) + 1) for the rank that owns the
+ * whole periodic box; the headers misa-md_amd/host/comm/{domain,comm}.h hold
+ * the C++ API (Region, BccDomain::Builder, fwCommLocalRegion, Packer,
+ * neiSendReceive) and misa-md_amd/host/logs/logs.h the kiwi::logs calls.
+ * Regions are {x_low, y_low, z_low, x_high, y_high, z_high}. */
+typedef struct misa_domain {
+  int64_t phase_space[3];
+  double lattice_const;
+  double cutoff_radius_factor;
+  int64_t cut_lattice;
+  int32_t grid_size[3];
+  int32_t grid_coord[3];
+  int32_t rank_id_neighbours[6]; /* [dimension][direction] */
+  int32_t pad_;
+  double meas_global_length[3];
+  double meas_global_region[6];
+  double meas_sub_box_region[6];
+  int64_t sub_box_lattice_size[3];
+  int64_t lattice_size_ghost[3];
+  int64_t ghost_extended_lattice_size[3];
+  int64_t sub_box_lattice_region[6];
+  int64_t ghost_ext_lattice_region[6];
+  int64_t dbx_sub_box_lattice_size[3];
+  int64_t dbx_lattice_size_ghost[3];
+  int64_t dbx_ghost_extended_lattice_size[3];
+  int64_t dbx_sub_box_lattice_region[6];
+  int64_t dbx_ghost_ext_lattice_region[6];
+} misa_domain;
+int misa_host_domain(const misa_geom* g, misa_domain* out);
+/* comm::fwCommLocalRegion: the slab of the ghost-extended array (x doubled,
+ * indices from 0) sent to the neighbour in (dimension, direction). */
+int misa_host_fw_region(const misa_geom* g, int dimension, int direction, int64_t region[6]);
+/* AtomList::exchangeAtomFirst (atom_list.cpp:25-49) at one rank on a host
+ * AtomElement array: every ghost slot gets the type and the shifted position
+ * of its periodic image (LatPackerFirst, lat_particle_packer.cpp:22-160). */
+int misa_host_exchange_atoms(const misa_geom* g, void* atoms);
+
 /* ---- frontend I/O: atom dumps (frontend/io), one rank ----
  *
  * Copy mode (AtomDump, frontend/io/atom_dump.cpp:44-101, with

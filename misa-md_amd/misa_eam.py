@@ -52,6 +52,22 @@ DUMP_DEBUG = 0
 DUMP_COPY = 1
 
 
+class MisaDomain(ctypes.Structure):
+    """comm::BccDomain of one rank (regions as x_low, y_low, z_low, x_high, y_high, z_high)."""
+    _fields_ = [("phase_space", ctypes.c_int64 * 3), ("lattice_const", ctypes.c_double),
+                ("cutoff_radius_factor", ctypes.c_double), ("cut_lattice", ctypes.c_int64),
+                ("grid_size", ctypes.c_int32 * 3), ("grid_coord", ctypes.c_int32 * 3),
+                ("rank_id_neighbours", ctypes.c_int32 * 6), ("pad_", ctypes.c_int32),
+                ("meas_global_length", ctypes.c_double * 3), ("meas_global_region", ctypes.c_double * 6),
+                ("meas_sub_box_region", ctypes.c_double * 6), ("sub_box_lattice_size", ctypes.c_int64 * 3),
+                ("lattice_size_ghost", ctypes.c_int64 * 3), ("ghost_extended_lattice_size", ctypes.c_int64 * 3),
+                ("sub_box_lattice_region", ctypes.c_int64 * 6), ("ghost_ext_lattice_region", ctypes.c_int64 * 6),
+                ("dbx_sub_box_lattice_size", ctypes.c_int64 * 3), ("dbx_lattice_size_ghost", ctypes.c_int64 * 3),
+                ("dbx_ghost_extended_lattice_size", ctypes.c_int64 * 3),
+                ("dbx_sub_box_lattice_region", ctypes.c_int64 * 6),
+                ("dbx_ghost_ext_lattice_region", ctypes.c_int64 * 6)]
+
+
 class MisaMdReport(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_uint64), ("phy_time", ctypes.c_double), ("stages_run", ctypes.c_int64),
                 ("collisions", ctypes.c_int64), ("rescales", ctypes.c_int64), ("dump_frames", ctypes.c_int64),
@@ -149,6 +165,10 @@ _SIGS = {
     "misa_config_parse": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "misa_config_parse_text": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "misa_md_simulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int32, ctypes.c_void_p]),
+    "misa_host_domain": (ctypes.c_int, [ctypes.POINTER(MisaGeom), ctypes.c_void_p]),
+    "misa_host_fw_region": (ctypes.c_int, [ctypes.POINTER(MisaGeom), ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_int64)]),
+    "misa_host_exchange_atoms": (ctypes.c_int, [ctypes.POINTER(MisaGeom), ctypes.c_void_p]),
 }
 
 
@@ -545,3 +565,31 @@ def md_simulate(cfg, device=0, precision=PREC_FP64):
     if rc != 0:
         raise MisaError("misa_md_simulate (%d)" % rc)
     return rep
+
+
+# ---- host mirror of libcomm's one-rank domain ----
+
+def host_domain(cells, lattice_const=A_FE, rc_factor=RC_FACTOR):
+    lib = load()
+    g = geom(cells, lattice_const, rc_factor)
+    d = MisaDomain()
+    if lib.misa_host_domain(ctypes.byref(g), ctypes.byref(d)) != 0:
+        raise MisaError("misa_host_domain")
+    return d
+
+
+def host_fw_region(cells, dimension, direction, lattice_const=A_FE, rc_factor=RC_FACTOR):
+    lib = load()
+    g = geom(cells, lattice_const, rc_factor)
+    r = (ctypes.c_int64 * 6)()
+    if lib.misa_host_fw_region(ctypes.byref(g), dimension, direction, r) != 0:
+        raise MisaError("misa_host_fw_region")
+    return list(r)
+
+
+def host_exchange_atoms(atoms, cells, lattice_const=A_FE, rc_factor=RC_FACTOR):
+    """exchangeAtomFirst at one rank, in place on a ghost-extended array."""
+    lib = load()
+    g = geom(cells, lattice_const, rc_factor)
+    if lib.misa_host_exchange_atoms(ctypes.byref(g), _ptr(atoms)) != 0:
+        raise MisaError("misa_host_exchange_atoms")
