@@ -185,7 +185,7 @@ int syncAndCheck(misa_handle* h) {
 int allocInter(misa_handle* h, int64_t cap) {
   InterDev& in = h->s.in;
   if (h->interBlock) {
-    hipFree(h->interBlock);
+    (void)hipFree(h->interBlock);
     h->interBlock = nullptr;
   }
   const int64_t ecap = 4 * cap;
@@ -260,15 +260,15 @@ struct Timer {
     e0 = h->events.size();
     h->event_cls.push_back(c);
     hipEvent_t a, b;
-    hipEventCreate(&a);
-    hipEventCreate(&b);
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
     h->events.push_back(a);
     h->events.push_back(b);
-    hipEventRecord(a, h->stream);
+    (void)hipEventRecord(a, h->stream);
   }
   ~Timer() {
     if (!h->profile) return;
-    hipEventRecord(h->events[e0 + 1], h->stream);
+    (void)hipEventRecord(h->events[e0 + 1], h->stream);
   }
 };
 
@@ -348,18 +348,18 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
     return MISA_E_HIP;
   }
   for (int k = 0; k < 3; k++) {
-    hipMemset(s.x[0][k], 0, slots * 8);
-    hipMemset(s.x[1][k], 0, slots * 8);
-    hipMemset(s.v[k], 0, slots * 8);
-    hipMemset(s.f[k], 0, slots * 8);
+    (void)hipMemset(s.x[0][k], 0, slots * 8);
+    (void)hipMemset(s.x[1][k], 0, slots * 8);
+    (void)hipMemset(s.v[k], 0, slots * 8);
+    (void)hipMemset(s.f[k], 0, slots * 8);
   }
-  hipMemset(s.rho, 0, slots * 8);
-  hipMemset(s.df64, 0, slots * 8);
-  hipMemset(s.rec[0], 0, slots * sizeof(float4));
-  hipMemset(s.rec[1], 0, slots * sizeof(float4));
-  hipMemset(s.type, 0xff, slots);
-  hipMemset(s.id, 0, slots * 8);
-  hipMemset(s.ctl, 0, sizeof(Ctl));
+  (void)hipMemset(s.rho, 0, slots * 8);
+  (void)hipMemset(s.df64, 0, slots * 8);
+  (void)hipMemset(s.rec[0], 0, slots * sizeof(float4));
+  (void)hipMemset(s.rec[1], 0, slots * sizeof(float4));
+  (void)hipMemset(s.type, 0xff, slots);
+  (void)hipMemset(s.id, 0, slots * 8);
+  (void)hipMemset(s.ctl, 0, sizeof(Ctl));
   if (hipDeviceSynchronize() != hipSuccess) {
     misa_destroy(h);
     return MISA_E_HIP;
@@ -370,14 +370,14 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
 
 int misa_destroy(misa_handle* h) {
   if (!h) return MISA_E_ARG;
-  hipSetDevice(h->device);
-  if (h->stream) hipStreamSynchronize(h->stream);
-  for (void* p : h->allocs) hipFree(p);
-  if (h->aos_dev) hipFree(h->aos_dev);
-  if (h->interBlock) hipFree(h->interBlock);
-  if (h->interAos) hipFree(h->interAos);
-  for (auto e : h->events) hipEventDestroy(e);
-  if (h->stream) hipStreamDestroy(h->stream);
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void* p : h->allocs) (void)hipFree(p);
+  if (h->aos_dev) (void)hipFree(h->aos_dev);
+  if (h->interBlock) (void)hipFree(h->interBlock);
+  if (h->interAos) (void)hipFree(h->interAos);
+  for (auto e : h->events) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return MISA_OK;
 }
@@ -861,15 +861,15 @@ int finishSteps(misa_handle* h, int64_t nsteps, hipEvent_t e0, hipEvent_t e1) {
   const int rc = syncAndCheck(h);
   float ms = 0;
   if (e0 && e1) {
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
   }
   h->timings[0] = h->timings[1] = h->timings[2] = 0;
   h->launches[0] = h->launches[1] = h->launches[2] = 0;
   for (size_t k = 0; k < h->event_cls.size(); k++) {
     float t = 0;
-    hipEventElapsedTime(&t, h->events[2 * k], h->events[2 * k + 1]);
+    (void)hipEventElapsedTime(&t, h->events[2 * k], h->events[2 * k + 1]);
     h->timings[h->event_cls[k]] += t;
     h->launches[h->event_cls[k]]++;
   }
@@ -879,7 +879,7 @@ int finishSteps(misa_handle* h, int64_t nsteps, hipEvent_t e0, hipEvent_t e1) {
 }
 
 void clearEvents(misa_handle* h) {
-  for (auto e : h->events) hipEventDestroy(e);
+  for (auto e : h->events) (void)hipEventDestroy(e);
   h->events.clear();
   h->event_cls.clear();
 }
@@ -1079,7 +1079,7 @@ std::vector<uint64_t> batchKey(const misa_batch* b) {
 // forked from and joined back into the origin stream.
 int captureBatch(misa_batch* b, const std::vector<StepPlan>& plans) {
   if (b->exec) {
-    hipGraphExecDestroy(b->exec);
+    (void)hipGraphExecDestroy(b->exec);
     b->exec = nullptr;
   }
   BHIP(b, hipStreamBeginCapture(b->origin, hipStreamCaptureModeRelaxed));
@@ -1098,7 +1098,7 @@ int captureBatch(misa_batch* b, const std::vector<StepPlan>& plans) {
   if (e != hipSuccess) return bfail(b, MISA_E_HIP, std::string("capturing the step graph: ") + hipGetErrorString(e));
   if (e2 != hipSuccess) return bfail(b, MISA_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
   const hipError_t e3 = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);
   if (e3 != hipSuccess) return bfail(b, MISA_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
   b->key = batchKey(b);
   return MISA_OK;
@@ -1141,14 +1141,14 @@ int misa_batch_create(misa_handle* const* handles, int n, misa_batch** out) {
 
 int misa_batch_destroy(misa_batch* b) {
   if (!b) return MISA_E_ARG;
-  hipSetDevice(b->device);
-  if (b->origin) hipStreamSynchronize(b->origin);
-  if (b->exec) hipGraphExecDestroy(b->exec);
+  (void)hipSetDevice(b->device);
+  if (b->origin) (void)hipStreamSynchronize(b->origin);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
   for (hipEvent_t e : b->done)
-    if (e) hipEventDestroy(e);
-  if (b->fork) hipEventDestroy(b->fork);
-  if (b->after) hipEventDestroy(b->after);
-  if (b->origin) hipStreamDestroy(b->origin);
+    if (e) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->after) (void)hipEventDestroy(b->after);
+  if (b->origin) (void)hipStreamDestroy(b->origin);
   delete b;
   return MISA_OK;
 }

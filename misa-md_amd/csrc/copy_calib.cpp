@@ -21,7 +21,7 @@ int main(int argc, char** argv) {
   for (int rep = 0; rep < 3; rep++) copyKernel<<<4096, 256>>>(a, b, n);
   if (hipDeviceSynchronize() != hipSuccess) return 1;
   std::printf("copied %zu bytes per launch, 3 launches\n", n * sizeof(double4));
-  hipFree(a);
-  hipFree(b);
+  (void)hipFree(a);
+  (void)hipFree(b);
   return 0;
 }

@@ -47,7 +47,6 @@ namespace {
 using namespace site;
 
 __device__ __forceinline__ double rsqT(double x) { return rsqrtRefined(x); }
-__device__ __forceinline__ float rsqT(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // ---------------------------------------------------------------------------
 // pass 1: rho and F'
