@@ -148,6 +148,14 @@ int misa_set_timestep(misa_handle* h, double dt);
  * inter atoms, computed on the device from the velocities at the start of the
  * step; max_disp = 0 returns to the fixed step of misa_set_timestep. */
 int misa_set_adaptive_dt(misa_handle* h, double dt_max, double max_disp);
+/* Neighbour skin of the mixed density pass (Angstrom; default 0.2, 0 turns
+ * it off; no counterpart in the reference, whose lists are rebuilt every
+ * step).  A full scan of the pruned offsets also records, per wavefront, the
+ * offsets within cutoff + skin; until the lattice atoms' summed per-step
+ * displacement reaches skin / 2 (or a refill, upload or setter change), the
+ * density pass visits only those.  The pairs and their order do not change,
+ * so neither do the results. */
+int misa_set_neighbour_skin(misa_handle* h, double skin);
 /* Simulated time (ps) since the last upload, and the length of the last step. */
 int misa_sim_time(misa_handle* h, double* t_ps, double* dt_last);
 /* Inter-atom list capacity (default 65536 entries). */

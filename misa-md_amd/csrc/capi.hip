@@ -47,6 +47,7 @@ struct misa_handle {
   bool state_ready = false;
   bool forces_ready = false;
   uint64_t gen = 0;  // bumped by every change a captured step graph would bake in
+  uint64_t skin_gen = ~0ull;  // gen at which the density pass's skin masks were last invalidated
   AtomElement* aos_dev = nullptr;
   std::vector<void*> allocs;
   bool profile = false;
@@ -335,6 +336,7 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
   if (rc == MISA_OK) rc |= devAlloc(h, &s.rec[1], slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.dl, 2 * kMaxLatOffsets);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.wmask, static_cast<size_t>(2 * 4 * 16 * G.W.bpx));
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.smask, static_cast<size_t>(2 * 4 * 16 * G.W.bpx));
   if (rc == MISA_OK) rc |= devAlloc(h, &s.type, slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.id, slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.ctl, 1);
@@ -748,6 +750,13 @@ int misa_set_adaptive_dt(misa_handle* h, double dt_max, double max_disp) {
   return MISA_OK;
 }
 
+int misa_set_neighbour_skin(misa_handle* h, double skin) {
+  if (!h || !(skin >= 0.0) || skin > 1.0) return MISA_E_ARG;
+  h->s.skin = skin;
+  h->gen++;
+  return MISA_OK;
+}
+
 int misa_sim_time(misa_handle* h, double* t_ps, double* dt_last) {
   if (!h || !t_ps) return MISA_E_ARG;
   HIPCHECK(h, hipSetDevice(h->device));
@@ -832,6 +841,14 @@ StepPlan planOf(misa_handle* h) {
   return p;
 }
 
+// Any setter change (offsets, potential, shape, precision, ...) may leave
+// the density pass's skin masks stale: the next decide scans in full.
+void enqRescanIfChanged(misa_handle* h) {
+  if (h->skin_gen == h->gen) return;
+  launchRescan(h->s, h->stream);
+  h->skin_gen = h->gen;
+}
+
 void enqFirstAndDecide(misa_handle* h, const StepPlan& p, bool lattice, bool vmax_done = false) {
   {
     Timer t(h, 2);
@@ -899,6 +916,7 @@ int misa_step(misa_handle* h, int64_t nsteps) {
   HIPCHECK(h, hipEventCreate(&e0));
   HIPCHECK(h, hipEventCreate(&e1));
   HIPCHECK(h, hipEventRecord(e0, h->stream));
+  enqRescanIfChanged(h);
   enqFirstAndDecide(h, p, true);
   for (int64_t i = 0; i < nsteps; i++) enqBody(h, p, i == nsteps - 1);
   HIPCHECK(h, hipGetLastError());
@@ -1172,7 +1190,10 @@ int misa_batch_step(misa_batch* b, int64_t nsteps, int64_t* graph_replays) {
     plans.push_back(planOf(h));
   }
   // prologue of every box on its own stream
-  for (size_t i = 0; i < n; i++) enqFirstAndDecide(b->hs[i], plans[i], true);
+  for (size_t i = 0; i < n; i++) {
+    enqRescanIfChanged(b->hs[i]);
+    enqFirstAndDecide(b->hs[i], plans[i], true);
+  }
   const int64_t pairs = (nsteps - 1) / 2;
   int rc = MISA_OK;
   if (pairs > 0) {

@@ -72,8 +72,18 @@ struct Ctl {
   double dt;            // adaptive step: length of the current step (ps)
   double t_sim;         // adaptive step: simulated time accumulated on the device (ps)
   unsigned long long vmax2;  // adaptive step: max |v|^2 (bits of a non-negative double)
-  int64_t pad[5];
+  // neighbour skin of the mixed density pass: largest displacement of a
+  // lattice atom summed over the drifts since the last full scan (A), the
+  // current drift's max |dx|^2 (bits), whether this step's density pass
+  // visits only the offsets within cutoff + skin of the last full scan, and
+  // a request for a full scan
+  double disp_acc;
+  unsigned long long dispmax2;
+  int32_t skin_mode;
+  int32_t rescan;
+  int64_t pad[2];
 };
+static_assert(sizeof(Ctl) == 15 * 8, "Ctl layout");
 
 // Masses and the force-to-velocity unit factor (atom_types.h:17-19,
 // pre_define.h:11-21) for the kernels that derive the half-kick factors from

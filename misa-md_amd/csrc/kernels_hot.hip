@@ -33,6 +33,8 @@
 // cubic is fetched separately only for unlike pairs.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "device_common.hpp"
 #include "inter_common.hpp"
 #include "resident.hpp"
@@ -150,7 +152,7 @@ __device__ __forceinline__ void verletTail(const Geom& G, int64_t s, int p, int 
                                            int ei, double xi0, double xi1, double xi2, double f0, double f1,
                                            double f2, XPtr xn, XPtr v, XPtr f, const uint64_t* __restrict__ id,
                                            const VerletParams& vp, const StageArgs& st, Ctl* ctl,
-                                           unsigned long long& vb) {
+                                           unsigned long long& vb, unsigned long long& db) {
 #pragma clang fp contract(off)
   const StepDt sd = stepDt(vp, ei);
   const double c = sd.c;
@@ -179,6 +181,7 @@ __device__ __forceinline__ void verletTail(const Geom& G, int64_t s, int p, int 
   const double y0 = xi0 + sd.dt * v0;
   const double y1 = xi1 + sd.dt * v1;
   const double y2 = xi2 + sd.dt * v2;
+  db = speed2Bits(y0 - xi0, y1 - xi1, y2 - xi2);
   if (leftSite(G, p, cx, cy, cz, y0, y1, y2)) {
     stageInter(st, ctl, refIndex(G, p, cx, cy, cz), s, id[s], ti, y0, y1, y2, v0, v1, v2);
     v0 = v1 = v2 = 0.0;
@@ -208,6 +211,7 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
                                                  XPtr v, XPtr f, const uint64_t* __restrict__ id,
                                                  VerletParams vp, StageArgs st, Ctl* ctl, InterView iv) {
   unsigned long long vb = 0;  // |v|^2 bits after the secondstep (MODE 0)
+  unsigned long long db = 0;  // |dx|^2 bits of the drift (MODE 1)
   [&] {
   int64_t s;
   int p, cx, cy, cz;
@@ -334,9 +338,10 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
   }
   double f0, f1, f2;
   sum3Value<ACC>(acc, f0, f1, f2);
-  verletTail<MODE>(G, s, p, cx, cy, cz, ti, ei, xi0, xi1, xi2, f0, f1, f2, xn, v, f, id, vp, st, ctl, vb);
+  verletTail<MODE>(G, s, p, cx, cy, cz, ti, ei, xi0, xi1, xi2, f0, f1, f2, xn, v, f, id, vp, st, ctl, vb, db);
   }();
   if (MODE == 0 && vp.vmax) waveMaxTo(vb, vp.vmax);
+  if (MODE == 1) waveMaxTo(db, &ctl->dispmax2);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,17 +369,26 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
   return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+// The density pass either scans every pruned offset and records, besides
+// the offsets in range, those within cutoff + skin for some lane (smask), or
+// (Ctl::skin_mode, set by decide while no atom can have crossed the skin)
+// visits only the offsets of the last scan's smask: the same pairs in the
+// same order, so the two give the same sums bit for bit.
 template <int ACC, int U>
 __global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<float> tb, CXPtr x,
                                                   const int8_t* __restrict__ type, float4* rec,
                                                   const float4* __restrict__ dl, double* __restrict__ rho_out,
-                                                  uint64_t* __restrict__ wmask, InterView iv) {
+                                                  uint64_t* __restrict__ wmask, uint64_t* __restrict__ smask,
+                                                  float rcs2, InterView iv) {
   int64_t s;
   int p, cx, cy, cz;
   if (!siteOfThread(G, s, p, cx, cy, cz)) return;
   const int8_t ti = type[s];
+  const bool skin = __builtin_amdgcn_readfirstlane(iv.ctl->skin_mode) != 0;
+  const uint32_t wv = waveId();
   Sum1<ACC> acc;
   uint64_t w0 = 0, w1 = 0;  // offsets in range for some lane (uniform among the lanes that gather)
+  uint64_t s0 = 0, s1 = 0;  // offsets within cutoff + skin for some lane (full scan)
   if (!isVacant(ti)) {
     const float4 me = rec[s];
     const Rsrc rr = rsrcBelow(rec, O.lo);
@@ -384,27 +398,66 @@ __global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<f
     const float4* D = dl + p * kMaxLatOffsets;
     const float rc2 = static_cast<float>(G.rc2);
     const int nk = O.n[p];
-    auto group = [&](auto UC, int k0) {
+    auto group = [&](auto UC, const int* ks) {
       constexpr int V = decltype(UC)::value;
       float4 nb[V], d[V];
-      bool in[V];
+      bool in[V], sk[V];
 #pragma unroll
       for (int u = 0; u < V; ++u) {
-        nb[u] = ldRec4(rr, o16, sh[k0 + u] * 16u);
-        d[u] = D[k0 + u];
+        nb[u] = ldRec4(rr, o16, sh[ks[u]] * 16u);
+        d[u] = D[ks[u]];
       }
-      rhoGroupM<ACC, V>(me, nb, d, rc2, rtab, tb, acc, in);
+      rhoGroupM<ACC, V>(me, nb, d, rc2, rcs2, rtab, tb, acc, in, sk);
 #pragma unroll
-      for (int u = 0; u < V; ++u)
+      for (int u = 0; u < V; ++u) {
+        const int k = ks[u];
         if (__ballot(in[u])) {
-          const int k = k0 + u;
           if (k < 64) w0 |= 1ull << k;
           else w1 |= 1ull << (k - 64);
         }
+        if (!skin && __ballot(sk[u])) {
+          if (k < 64) s0 |= 1ull << k;
+          else s1 |= 1ull << (k - 64);
+        }
+      }
     };
-    int k = 0;
-    for (; k + U <= nk; k += U) group(std::integral_constant<int, U>{}, k);
-    for (; k < nk; ++k) group(std::integral_constant<int, 1>{}, k);
+    if (!skin) {
+      int ks[U];
+      int k = 0;
+      for (; k + U <= nk; k += U) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) ks[u] = k + u;
+        group(std::integral_constant<int, U>{}, ks);
+      }
+      for (; k < nk; ++k) {
+        ks[0] = k;
+        group(std::integral_constant<int, 1>{}, ks);
+      }
+    } else {
+      uint64_t b0 = uniform64(smask[2 * wv]), b1 = uniform64(smask[2 * wv + 1]);
+      int left = __builtin_popcountll(b0) + __builtin_popcountll(b1);
+      for (; left > 0;) {
+        const int V = left >= U ? U : 1;
+        int ks[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (u < V) {
+            if (b0) {
+              ks[u] = __builtin_ctzll(b0);
+              b0 &= b0 - 1;
+            } else {
+              ks[u] = 64 + __builtin_ctzll(b1);
+              b1 &= b1 - 1;
+            }
+          } else {
+            ks[u] = 0;
+          }
+        }
+        if (V == U) group(std::integral_constant<int, U>{}, ks);
+        else group(std::integral_constant<int, 1>{}, ks);
+        left -= V;
+      }
+    }
   }
   // one lane that gathered writes the wavefront's offsets
   const uint64_t gathered = __ballot(!isVacant(ti));
@@ -412,8 +465,15 @@ __global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<f
     const int leader = __builtin_ctzll(gathered);
     const uint64_t a0 = __shfl(w0, leader, 64), a1 = __shfl(w1, leader, 64);
     if (static_cast<int>(threadIdx.x & 63) == leader) {
-      wmask[2 * waveId()] = a0;
-      wmask[2 * waveId() + 1] = a1;
+      wmask[2 * wv] = a0;
+      wmask[2 * wv + 1] = a1;
+    }
+    if (!skin) {
+      const uint64_t c0 = __shfl(s0, leader, 64), c1 = __shfl(s1, leader, 64);
+      if (static_cast<int>(threadIdx.x & 63) == leader) {
+        smask[2 * wv] = c0;
+        smask[2 * wv + 1] = c1;
+      }
     }
   }
   rhoTailM<ACC>(G, s, p, cx, cy, cz, ti, acc, x, type, tb, iv, rec, rho_out);
@@ -427,6 +487,7 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
                                                   const uint64_t* __restrict__ id, VerletParams vp, StageArgs st,
                                                   Ctl* ctl, InterView iv) {
   unsigned long long vb = 0;  // |v|^2 bits after the secondstep (MODE 0)
+  unsigned long long db = 0;  // |dx|^2 bits of the drift (MODE 1)
   [&] {
   int64_t s;
   int p, cx, cy, cz;
@@ -484,9 +545,10 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
     left -= V;
   }
   forceTailM<ACC, MODE>(G, s, p, cx, cy, cz, ti, ei, dfi, acc, x, xn, type, rec, recn, v, f, id, vp, st, ctl, tb,
-                        iv, vb);
+                        iv, vb, db);
   }();
   if (MODE == 0 && vp.vmax) waveMaxTo(vb, vp.vmax);
+  if (MODE == 1) waveMaxTo(db, &ctl->dispmax2);
 }
 
 constexpr int kUDouble = 2;
@@ -502,8 +564,11 @@ void launchRhoDf(const DevState& s, const LatOffsets& o, const Tables<double>& t
   if (!mixed)
     kRhoDf<double, 0, 4><<<grid, kBlock, 0, st>>>(s.G, o, t64, x, s.type, s.df64, rho, iv);
   else {
-#define MISA_RHO_M(A, U) \
-  kRhoDfM<A, U><<<grid, kBlock, 0, st>>>(s.G, o, t32, x, s.type, s.rec[s.cur], s.dl, rho, s.wmask, iv)
+    const double rcs = std::sqrt(s.G.rc2) + s.skin;
+    const float rcs2 = static_cast<float>(rcs * rcs);
+#define MISA_RHO_M(A, U)                                                                                   \
+  kRhoDfM<A, U><<<grid, kBlock, 0, st>>>(s.G, o, t32, x, s.type, s.rec[s.cur], s.dl, rho, s.wmask, s.smask, \
+                                        rcs2, iv)
     const bool u8 = s.uRho == 8;
     if (acc == 0) {
       if (u8) MISA_RHO_M(0, 8);

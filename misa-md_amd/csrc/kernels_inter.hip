@@ -39,6 +39,9 @@ constexpr int kDT = 1024;  // threads of the single-workgroup decide kernel
 constexpr int kIT = 256;   // threads per block of the per-atom kernels
 constexpr int kIB = kInterBlocks;  // blocks of the per-atom kernels (grid-stride)
 
+// Rounding allowance of the neighbour skin's displacement bound (A).
+constexpr double kSkinMargin = 1e-3;
+
 struct DecideArgs {
   Geom G;
   int mode;
@@ -50,6 +53,7 @@ struct DecideArgs {
   float4* lrec;  // mixed precision: site records of the current buffer, else null
   InterDev in;
   Ctl* ctl;
+  double skin;  // neighbour skin of the mixed density pass (0: off)
 };
 
 // Exclusive block-wide prefix sum of v (1024 threads); `total` gets the sum.
@@ -172,6 +176,7 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
   if (tid == 0) sFar = static_cast<int>(min(A.ctl->n_far, in.cap));
   __syncthreads();
   unsigned long long* err = reinterpret_cast<unsigned long long*>(&A.ctl->err_flags);
+  int refills = 0;
 
   if (A.mode == 0) {
     // P1: append the new inter atoms in the order of the sites they left
@@ -253,6 +258,7 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
       kept += tot;
     }
     __syncthreads();
+    refills = nl - kept;
     nl = kept;
     fromScratch(in, nl);
   }
@@ -439,8 +445,24 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
     A.ctl->n_prev = ne;
     A.ctl->n_far_prev = nfar;
     A.ctl->n_far = 0;
+    // Neighbour skin of the mixed density pass.  Since the last full scan no
+    // lattice atom has moved farther than disp_acc, so no pair has come closer
+    // by more than 2 disp_acc: an offset beyond cutoff + skin for every lane
+    // then is beyond the cutoff now while disp_acc < skin / 2.  A refill puts
+    // a new atom on a site, and the first force pass (mode 1) follows an
+    // upload: both need a full scan.
+    Ctl* c = A.ctl;
+    const double acc = c->disp_acc + sqrt(__longlong_as_double(static_cast<long long>(c->dispmax2)));
+    const bool full = A.mode != 0 || !(A.skin > 0.0) || c->rescan != 0 || refills > 0 ||
+                      acc >= 0.5 * A.skin - kSkinMargin;
+    c->skin_mode = full ? 0 : 1;
+    c->disp_acc = full ? 0.0 : acc;
+    c->dispmax2 = 0ull;
+    c->rescan = 0;
   }
 }
+
+__global__ void kRescan(Ctl* ctl) { ctl->rescan = 1; }
 
 // Lattice atoms farther than 0.2a from their site (the test of atom::decide,
 // atom.cpp:33-49) -> far list, before an index built without a decide.
@@ -824,8 +846,11 @@ void launchDecide(const DevState& s, int mode, bool mixed, hipStream_t st) {
   A.lrec = mixed ? s.rec[s.cur] : nullptr;
   A.in = s.in;
   A.ctl = s.ctl;
+  A.skin = mixed ? s.skin : 0.0;
   kDecide<<<1, kDT, 0, st>>>(A);
 }
+
+void launchRescan(const DevState& s, hipStream_t st) { kRescan<<<1, 1, 0, st>>>(s.ctl); }
 
 void launchInterRho(const DevState& s, const Tables<double>& t64, const Tables<float>& t32, bool mixed,
                     hipStream_t st) {

@@ -40,6 +40,8 @@ __global__ __launch_bounds__(kBlock) void kFirst(Geom G, CXPtr x, XPtr xn, const
                                                  XPtr v, CXPtr f, const uint64_t* __restrict__ id, VerletParams vp,
                                                  StageArgs st, Ctl* ctl, float4* __restrict__ recn) {
 #pragma clang fp contract(off)  // the half kick and drift round as newton_motion.cpp does
+  unsigned long long db = 0;  // |dx|^2 bits of the drift
+  [&] {
   int64_t s;
   int p, cx, cy, cz;
   if (!siteOfThread(G, s, p, cx, cy, cz)) return;
@@ -54,6 +56,7 @@ __global__ __launch_bounds__(kBlock) void kFirst(Geom G, CXPtr x, XPtr xn, const
     y0 = y0 + sd.dt * v0;
     y1 = y1 + sd.dt * v1;
     y2 = y2 + sd.dt * v2;
+    db = speed2Bits(y0 - x.c[0][s], y1 - x.c[1][s], y2 - x.c[2][s]);
     if (leftSite(G, p, cx, cy, cz, y0, y1, y2)) {
       stageInter(st, ctl, refIndex(G, p, cx, cy, cz), s, id[s], ti, y0, y1, y2, v0, v1, v2);
       v0 = v1 = v2 = 0.0;
@@ -75,6 +78,8 @@ __global__ __launch_bounds__(kBlock) void kFirst(Geom G, CXPtr x, XPtr xn, const
     recn[s] = r;
     forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) { recn[q] = r; });
   }
+  }();
+  waveMaxTo(db, &ctl->dispmax2);
 }
 
 // Site records from x and type over every slot, ghosts included.

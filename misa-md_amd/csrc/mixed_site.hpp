@@ -31,10 +31,12 @@ __device__ __forceinline__ float dist2M(const float4& me, const float4& nb, cons
 }
 
 // rho_i += sum over the group of rho_{t_j}(r_ij) (pot eam::chargeDensity);
-// in[u] tells whether neighbour u is within the cutoff.
+// in[u] tells whether neighbour u is within the cutoff, sk[u] whether it is
+// within the cutoff plus the neighbour skin (r^2 < rcs2).
 template <int ACC, int V>
 __device__ __forceinline__ void rhoGroupM(const float4& me, const float4 (&nb)[V], const float4 (&d)[V], float rc2,
-                                          Rsrc rtab, const Tables<float>& tb, Sum1<ACC>& acc, bool (&in)[V]) {
+                                          float rcs2, Rsrc rtab, const Tables<float>& tb, Sum1<ACC>& acc,
+                                          bool (&in)[V], bool (&sk)[V]) {
 #pragma clang fp contract(off)  // explicit FMAs only: every instantiation rounds alike
   float pp[V];
   float4 cr[V];
@@ -43,6 +45,7 @@ __device__ __forceinline__ void rhoGroupM(const float4& me, const float4 (&nb)[V
     float d0, d1, d2;
     const float r2 = dist2M(me, nb[u], d[u], d0, d1, d2);
     in[u] = r2 < rc2;
+    sk[u] = r2 < rcs2;
     if (in[u]) {
       const int m = knotM(r2 * __builtin_amdgcn_rsqf(r2), tb.rdr, tb.nr, pp[u]);
       const uint32_t row = __umul24(static_cast<uint32_t>(recSpecies(nb[u].x)), static_cast<uint32_t>(tb.nr));
@@ -188,7 +191,7 @@ __device__ __forceinline__ void forceTailM(const Geom& G, int64_t s, int p, int 
                                            float4* __restrict__ recn, XPtr v, XPtr f,
                                            const uint64_t* __restrict__ id, const VerletParams& vp,
                                            const StageArgs& st, Ctl* ctl, const Tables<float>& tb,
-                                           const InterView& iv, unsigned long long& vb) {
+                                           const InterView& iv, unsigned long long& vb, unsigned long long& db) {
 #pragma clang fp contract(off)  // the half kicks and drift round as newton_motion.cpp does
   const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
   if (ti & (kNearInter | kWide)) {
@@ -241,6 +244,7 @@ __device__ __forceinline__ void forceTailM(const Geom& G, int64_t s, int p, int 
   const double y0 = xi0 + sd.dt * v0;
   const double y1 = xi1 + sd.dt * v1;
   const double y2 = xi2 + sd.dt * v2;
+  db = speed2Bits(y0 - xi0, y1 - xi1, y2 - xi2);
   if (leftSite(G, p, cx, cy, cz, y0, y1, y2)) {
     stageInter(st, ctl, refIndex(G, p, cx, cy, cz), s, id[s], ti, y0, y1, y2, v0, v1, v2);
     v0 = v1 = v2 = 0.0;

@@ -73,6 +73,11 @@ struct DevState {
   uint64_t* wmask = nullptr;
   // gathers in flight per wave in the mixed passes (chosen by misa_autotune)
   int uRho = 4, uForce = 4;
+  // neighbour skin of the mixed density pass (A; 0: every step scans all
+  // pruned offsets) and, per wavefront, the pruned offsets within cutoff +
+  // skin for some lane at the last full scan (2 words)
+  double skin = 0.2;
+  uint64_t* smask = nullptr;
   InterDev in;
   Ctl* ctl = nullptr;
   double* partials = nullptr;  // reduction scratch
@@ -110,15 +115,17 @@ int64_t launchThermo(const DevState& s, hipStream_t st);
 int64_t launchEnergy(const DevState& s, const LatOffsets& o, const Tables<double>& t64, hipStream_t st);
 void launchScaleV(const DevState& s, double factor, hipStream_t st);
 // Adaptive step: max |v|^2 over lattice and inter atoms into Ctl::vmax2, then
-// Ctl::dt = min(dt_max, max_disp / |v|max) and Ctl::t_sim += Ctl::dt.
-// dt = min(dt_max, max_disp / max|v|); with sites false the lattice sites'
-// maximum is already in Ctl::vmax2 (reduced by the secondstep pass).
+// Ctl::dt = min(dt_max, max_disp / max|v|) and Ctl::t_sim += Ctl::dt; with
+// sites false the lattice sites' maximum is already in Ctl::vmax2 (reduced
+// by the secondstep pass).
 void launchAdaptiveDt(const DevState& s, double dt_max, double max_disp, bool sites, hipStream_t st);
 // Inter-atom work (kernels_inter.hip).
 // decide mode 0: the step's ownership pass (append new inter atoms, vacancy
 // recombination, migration, ghost images, index, flags); mode 1: migration,
 // ghost images, index and flags only (first force, PKA step, upload).
 void launchDecide(const DevState& s, int mode, bool mixed, hipStream_t st);
+// Ask the next decide for a full scan of the mixed density pass.
+void launchRescan(const DevState& s, hipStream_t st);
 InterView interView(const DevState& s, bool mixed);
 // Lattice atoms farther than 0.2a from their site -> far list (before a
 // mode-1 decide; in a step only refilled vacancies can be far).

@@ -165,6 +165,7 @@ _SIGS = {
     "misa_config_parse": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "misa_config_parse_text": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "misa_md_simulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int32, ctypes.c_void_p]),
+    "misa_set_neighbour_skin": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     "misa_host_domain": (ctypes.c_int, [ctypes.POINTER(MisaGeom), ctypes.c_void_p]),
     "misa_host_fw_region": (ctypes.c_int, [ctypes.POINTER(MisaGeom), ctypes.c_int, ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_int64)]),
@@ -301,6 +302,10 @@ class EamDevice:
 
     def set_adaptive_dt(self, dt_max, max_disp):
         self._check(self.lib.misa_set_adaptive_dt(self.h, dt_max, max_disp), "misa_set_adaptive_dt")
+
+    def set_neighbour_skin(self, skin):
+        """Skin (A) of the mixed density pass's offset masks; 0 turns it off."""
+        self._check(self.lib.misa_set_neighbour_skin(self.h, float(skin)), "misa_set_neighbour_skin")
 
     def sim_time(self):
         t, dt = ctypes.c_double(), ctypes.c_double()
