@@ -50,19 +50,98 @@ using namespace site;
 
 __device__ __forceinline__ double rsqT(double x) { return rsqrtRefined(x); }
 
+// |x_i - x_j|^2 in FP64 with one fixed rounding, so that the density and
+// force passes agree on which pairs are within the cutoff.
+__device__ __forceinline__ double dist2D(double d0, double d1, double d2) {
+  return __fma_rn(d2, d2, __fma_rn(d1, d1, __dmul_rn(d0, d0)));
+}
+
+// Index of the wavefront in the grid, in a scalar register (threadIdx.x >> 6
+// is the same in every lane of a wavefront).
+__device__ __forceinline__ uint32_t waveId() {
+  return blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+// A wavefront-uniform 64-bit value moved to scalar registers.
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Calls visit(ks, nv) for the set bits of (b0, b1) in increasing order, U at
+// a time, then one at a time (nv = U or 1; ks holds the offsets).
+template <int U, typename Visit>
+__device__ __forceinline__ void forMaskGroups(uint64_t b0, uint64_t b1, Visit visit) {
+  int left = __builtin_popcountll(b0) + __builtin_popcountll(b1);
+  for (; left > 0;) {
+    const int V = left >= U ? U : 1;
+    int ks[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (u < V) {
+        if (b0) {
+          ks[u] = __builtin_ctzll(b0);
+          b0 &= b0 - 1;
+        } else {
+          ks[u] = 64 + __builtin_ctzll(b1);
+          b1 &= b1 - 1;
+        }
+      } else {
+        ks[u] = 0;
+      }
+    }
+    visit(ks, V);
+    left -= V;
+  }
+}
+
+// Set bit k of the 128-bit offset mask (m0, m1).
+__device__ __forceinline__ void setBit(uint64_t& m0, uint64_t& m1, int k) {
+  if (k < 64) m0 |= 1ull << k;
+  else m1 |= 1ull << (k - 64);
+}
+
+// The leader (first gathering lane) writes the wavefront's masks.
+__device__ __forceinline__ void writeMasks(bool gathers, uint32_t wv, uint64_t w0, uint64_t w1, uint64_t* wmask,
+                                           bool full, uint64_t s0, uint64_t s1, uint64_t* smask) {
+  const uint64_t gathered = __ballot(gathers);
+  if (!gathered) return;
+  const int leader = __builtin_ctzll(gathered);
+  const uint64_t a0 = __shfl(w0, leader, 64), a1 = __shfl(w1, leader, 64);
+  const uint64_t c0 = __shfl(s0, leader, 64), c1 = __shfl(s1, leader, 64);
+  if (static_cast<int>(threadIdx.x & 63) == leader) {
+    wmask[2 * wv] = a0;
+    wmask[2 * wv + 1] = a1;
+    if (full) {
+      smask[2 * wv] = c0;
+      smask[2 * wv + 1] = c1;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // pass 1: rho and F'
+// FP64 and mixed density passes record, per wavefront, the pruned offsets
+// within the cutoff for some lane (wmask) for the force pass, which visits
+// only those.  A full scan also records the offsets within cutoff + skin
+// (smask); while decide finds that no atom can have crossed the skin
+// (Ctl::skin_mode) the density pass visits only those: the same pairs in the
+// same order, so the sums do not change.
 // ---------------------------------------------------------------------------
 template <typename T, int ACC, int U>
 __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T> tb, CXPtr x,
                                                  const int8_t* __restrict__ type, T* __restrict__ df,
-                                                 double* __restrict__ rho_out, InterView iv) {
+                                                 double* __restrict__ rho_out, uint64_t* __restrict__ wmask,
+                                                 uint64_t* __restrict__ smask, double rcs2, InterView iv) {
   int64_t s;
   int p, cx, cy, cz;
   if (!siteOfThread(G, s, p, cx, cy, cz)) return;
   const int8_t ti = type[s];
+  const bool skin = __builtin_amdgcn_readfirstlane(iv.ctl->skin_mode) != 0;
+  const uint32_t wv = waveId();
   T dfi = T(0);
   double rho = 0.0;
+  uint64_t w0 = 0, w1 = 0, s0 = 0, s1 = 0;
   if (!isVacant(ti)) {
     const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
     const Rsrc rx0 = rsrcBelow(x.c[0], O.lo), rx1 = rsrcBelow(x.c[1], O.lo), rx2 = rsrcBelow(x.c[2], O.lo);
@@ -72,13 +151,13 @@ __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T>
     const uint32_t* sh = O.sh[p];
     const int nk = O.n[p];
     Sum1<ACC> acc;
-    auto group = [&](auto UC, int k0) {
+    auto group = [&](auto UC, const int* ks) {
       constexpr int V = decltype(UC)::value;
       int8_t tj[V];
       double xj0[V], xj1[V], xj2[V];
 #pragma unroll
       for (int u = 0; u < V; ++u) {
-        const uint32_t so = sh[k0 + u];
+        const uint32_t so = sh[ks[u]];
         tj[u] = ldB(rt, o1, so);
         xj0[u] = ldD(rx0, o8, so * 8u);
         xj1[u] = ldD(rx1, o8, so * 8u);
@@ -89,9 +168,10 @@ __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T>
       T rec[V][kRhoRec];
 #pragma unroll
       for (int u = 0; u < V; ++u) {
-        const double d0 = xi0 - xj0[u], d1 = xi1 - xj1[u], d2 = xi2 - xj2[u];
-        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        const double r2 = dist2D(xi0 - xj0[u], xi1 - xj1[u], xi2 - xj2[u]);
         in[u] = !isVacant(tj[u]) && r2 < G.rc2;
+        if (!skin && __ballot(!isVacant(tj[u]) && r2 < rcs2)) setBit(s0, s1, ks[u]);
+        if (__ballot(in[u])) setBit(w0, w1, ks[u]);
         if (in[u]) {
           const T q2 = static_cast<T>(r2);
           const int m = knotOf(q2 * rsqT(q2), tb.rdr, tb.nr, false, pp[u]);
@@ -104,9 +184,24 @@ __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T>
       for (int u = 0; u < V; ++u)
         if (in[u]) acc.add(((rec[u][0] * pp[u] + rec[u][1]) * pp[u] + rec[u][2]) * pp[u] + rec[u][3]);
     };
-    int k = 0;
-    for (; k + U <= nk; k += U) group(std::integral_constant<int, U>{}, k);
-    for (; k < nk; ++k) group(std::integral_constant<int, 1>{}, k);
+    if (!skin) {
+      int ks[U];
+      int k = 0;
+      for (; k + U <= nk; k += U) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) ks[u] = k + u;
+        group(std::integral_constant<int, U>{}, ks);
+      }
+      for (; k < nk; ++k) {
+        ks[0] = k;
+        group(std::integral_constant<int, 1>{}, ks);
+      }
+    } else {
+      forMaskGroups<U>(uniform64(smask[2 * wv]), uniform64(smask[2 * wv + 1]), [&](const int* ks, int nv) {
+        if (nv == U) group(std::integral_constant<int, U>{}, ks);
+        else group(std::integral_constant<int, 1>{}, ks);
+      });
+    }
     if (ti & kNearInter) {  // inter atoms whose nearest site is within the full list (atom.cpp:224-240)
       const int ne = static_cast<int>(iv.ctl->n_inter + iv.ctl->n_ghost);
       const FullPlaneOffsets& full = *iv.full;
@@ -133,6 +228,7 @@ __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T>
     rho = acc.value();
     dfi = dEmbed(tb, species(ti), static_cast<T>(rho));
   }
+  writeMasks(!isVacant(ti), wv, w0, w1, wmask, !skin, s0, s1, smask);
   df[s] = dfi;
   if (rho_out) {
     rho_out[s] = rho;
@@ -208,8 +304,9 @@ __device__ __forceinline__ void verletTail(const Geom& G, int64_t s, int p, int 
 template <typename T, int ACC, int MODE, int U>
 __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T> tb, CXPtr x, XPtr xn,
                                                  const int8_t* __restrict__ type, const T* __restrict__ df,
-                                                 XPtr v, XPtr f, const uint64_t* __restrict__ id,
-                                                 VerletParams vp, StageArgs st, Ctl* ctl, InterView iv) {
+                                                 const uint64_t* __restrict__ wmask, XPtr v, XPtr f,
+                                                 const uint64_t* __restrict__ id, VerletParams vp, StageArgs st,
+                                                 Ctl* ctl, InterView iv) {
   unsigned long long vb = 0;  // |v|^2 bits after the secondstep (MODE 0)
   unsigned long long db = 0;  // |dx|^2 bits of the drift (MODE 1)
   [&] {
@@ -246,15 +343,14 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
   const uint32_t oT = static_cast<uint32_t>(s) * static_cast<uint32_t>(sizeof(T));
   const uint32_t pairBase = static_cast<uint32_t>(ei * 3) * static_cast<uint32_t>(tb.nr);
   const uint32_t* sh = O.sh[p];
-  const int nk = O.n[p];
   Sum3<ACC> acc;
-  auto group = [&](auto UC, int k0) {
+  auto group = [&](auto UC, const int* ks) {
     constexpr int V = decltype(UC)::value;
     int8_t tj[V];
     double d0[V], d1[V], d2[V];
 #pragma unroll
     for (int u = 0; u < V; ++u) {
-      const uint32_t so = sh[k0 + u];
+      const uint32_t so = sh[ks[u]];
       tj[u] = ldB(rt, o1, so);
       d0[u] = ldD(rx0, o8, so * 8u);
       d1[u] = ldD(rx1, o8, so * 8u);
@@ -268,10 +364,10 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
       d0[u] = xi0 - d0[u];
       d1[u] = xi1 - d1[u];
       d2[u] = xi2 - d2[u];
-      const double r2 = d0[u] * d0[u] + d1[u] * d1[u] + d2[u] * d2[u];
+      const double r2 = dist2D(d0[u], d1[u], d2[u]);
       in[u] = !isVacant(tj[u]) && r2 < G.rc2;
       if (in[u]) {
-        const uint32_t so = sh[k0 + u];
+        const uint32_t so = sh[ks[u]];
         const T q2 = static_cast<T>(r2);
         ri[u] = rsqT(q2);
         const int m = knotOf(q2 * ri[u], tb.rdr, tb.nr, false, pp[u]);
@@ -309,9 +405,12 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
       }
     }
   };
-  int k = 0;
-  for (; k + U <= nk; k += U) group(std::integral_constant<int, U>{}, k);
-  for (; k < nk; ++k) group(std::integral_constant<int, 1>{}, k);
+  // the offsets the density pass found in range for some lane of this wavefront
+  const uint32_t wv = waveId();
+  forMaskGroups<U>(uniform64(wmask[2 * wv]), uniform64(wmask[2 * wv + 1]), [&](const int* ks, int nv) {
+    if (nv == U) group(std::integral_constant<int, U>{}, ks);
+    else group(std::integral_constant<int, 1>{}, ks);
+  });
   if (ti & kNearInter) {  // forces from inter atoms on lattice neighbours (atom.cpp:397-422)
     const int ne = static_cast<int>(iv.ctl->n_inter + iv.ctl->n_ghost);
     const FullPlaneOffsets& full = *iv.full;
@@ -353,27 +452,8 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
 //   pairs within that distance of the cutoff, where the tabulated functions
 //   and their slopes vanish.
 // ---------------------------------------------------------------------------
-// Offsets of the pruned list (at most 128) within the cutoff for at least
-// one lane of a wavefront, recorded by the density pass for the force pass,
-// which then skips the others: both passes see the same positions and round
-// the distances the same way, so the skipped offsets have no pair.
-// Index of the wavefront in the grid, in a scalar register (threadIdx.x >> 6
-// is the same in every lane of a wavefront).
-__device__ __forceinline__ uint32_t waveId() {
-  return blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-}
-// A wavefront-uniform 64-bit value moved to scalar registers.
-__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
-  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
-  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
-  return (static_cast<uint64_t>(hi) << 32) | lo;
-}
-
-// The density pass either scans every pruned offset and records, besides
-// the offsets in range, those within cutoff + skin for some lane (smask), or
-// (Ctl::skin_mode, set by decide while no atom can have crossed the skin)
-// visits only the offsets of the last scan's smask: the same pairs in the
-// same order, so the two give the same sums bit for bit.
+// The masks of pass 1 work as in FP64: both passes see the same records and
+// round the distances the same way (dist2M), so a skipped offset has no pair.
 template <int ACC, int U>
 __global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<float> tb, CXPtr x,
                                                   const int8_t* __restrict__ type, float4* rec,
@@ -410,15 +490,8 @@ __global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<f
       rhoGroupM<ACC, V>(me, nb, d, rc2, rcs2, rtab, tb, acc, in, sk);
 #pragma unroll
       for (int u = 0; u < V; ++u) {
-        const int k = ks[u];
-        if (__ballot(in[u])) {
-          if (k < 64) w0 |= 1ull << k;
-          else w1 |= 1ull << (k - 64);
-        }
-        if (!skin && __ballot(sk[u])) {
-          if (k < 64) s0 |= 1ull << k;
-          else s1 |= 1ull << (k - 64);
-        }
+        if (__ballot(in[u])) setBit(w0, w1, ks[u]);
+        if (!skin && __ballot(sk[u])) setBit(s0, s1, ks[u]);
       }
     };
     if (!skin) {
@@ -434,6 +507,8 @@ __global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<f
         group(std::integral_constant<int, 1>{}, ks);
       }
     } else {
+      // (the loop of forMaskGroups written out: through the helper this
+      // kernel keeps the offsets on the stack)
       uint64_t b0 = uniform64(smask[2 * wv]), b1 = uniform64(smask[2 * wv + 1]);
       int left = __builtin_popcountll(b0) + __builtin_popcountll(b1);
       for (; left > 0;) {
@@ -459,23 +534,7 @@ __global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<f
       }
     }
   }
-  // one lane that gathered writes the wavefront's offsets
-  const uint64_t gathered = __ballot(!isVacant(ti));
-  if (gathered) {
-    const int leader = __builtin_ctzll(gathered);
-    const uint64_t a0 = __shfl(w0, leader, 64), a1 = __shfl(w1, leader, 64);
-    if (static_cast<int>(threadIdx.x & 63) == leader) {
-      wmask[2 * wv] = a0;
-      wmask[2 * wv + 1] = a1;
-    }
-    if (!skin) {
-      const uint64_t c0 = __shfl(s0, leader, 64), c1 = __shfl(s1, leader, 64);
-      if (static_cast<int>(threadIdx.x & 63) == leader) {
-        smask[2 * wv] = c0;
-        smask[2 * wv + 1] = c1;
-      }
-    }
-  }
+  writeMasks(!isVacant(ti), wv, w0, w1, wmask, !skin, s0, s1, smask);
   rhoTailM<ACC>(G, s, p, cx, cy, cz, ti, acc, x, type, tb, iv, rec, rho_out);
 }
 
@@ -511,8 +570,6 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
   Sum3<ACC> acc;
   // the offsets the density pass found in range for some lane of this wavefront
   const uint32_t wv = waveId();
-  uint64_t b0 = uniform64(wmask[2 * wv]), b1 = uniform64(wmask[2 * wv + 1]);
-  int left = __builtin_popcountll(b0) + __builtin_popcountll(b1);
   auto group = [&](auto UC, const int* ks) {
     constexpr int V = decltype(UC)::value;
     float4 nb[V], d[V];
@@ -523,27 +580,10 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
     }
     forceGroupM<ACC, V>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, acc);
   };
-  for (; left > 0;) {
-    const int V = left >= U ? U : 1;
-    int ks[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (u < V) {
-        if (b0) {
-          ks[u] = __builtin_ctzll(b0);
-          b0 &= b0 - 1;
-        } else {
-          ks[u] = 64 + __builtin_ctzll(b1);
-          b1 &= b1 - 1;
-        }
-      } else {
-        ks[u] = 0;
-      }
-    }
-    if (V == U) group(std::integral_constant<int, U>{}, ks);
+  forMaskGroups<U>(uniform64(wmask[2 * wv]), uniform64(wmask[2 * wv + 1]), [&](const int* ks, int nv) {
+    if (nv == U) group(std::integral_constant<int, U>{}, ks);
     else group(std::integral_constant<int, 1>{}, ks);
-    left -= V;
-  }
+  });
   forceTailM<ACC, MODE>(G, s, p, cx, cy, cz, ti, ei, dfi, acc, x, xn, type, rec, recn, v, f, id, vp, st, ctl, tb,
                         iv, vb, db);
   }();
@@ -561,10 +601,11 @@ void launchRhoDf(const DevState& s, const LatOffsets& o, const Tables<double>& t
   const unsigned grid = siteGrid(s.G);
   const InterView iv = interView(s, mixed);
   const CXPtr x = cxp(s.x[s.cur]);
+  const double rcs = std::sqrt(s.G.rc2) + s.skin;
   if (!mixed)
-    kRhoDf<double, 0, 4><<<grid, kBlock, 0, st>>>(s.G, o, t64, x, s.type, s.df64, rho, iv);
+    kRhoDf<double, 0, 4><<<grid, kBlock, 0, st>>>(s.G, o, t64, x, s.type, s.df64, rho, s.wmask, s.smask, rcs * rcs,
+                                                   iv);
   else {
-    const double rcs = std::sqrt(s.G.rc2) + s.skin;
     const float rcs2 = static_cast<float>(rcs * rcs);
 #define MISA_RHO_M(A, U)                                                                                   \
   kRhoDfM<A, U><<<grid, kBlock, 0, st>>>(s.G, o, t32, x, s.type, s.rec[s.cur], s.dl, rho, s.wmask, s.smask, \
@@ -592,8 +633,8 @@ void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t
   const StageArgs sa = stageOf(s);
   const InterView iv = interView(s, mixed);
 #define MISA_FORCE(M) \
-  kForce<double, 0, M, kUDouble><<<grid, kBlock, 0, st>>>(s.G, o, t64, x, xn, s.type, s.df64, xp(s.v), xp(s.f), \
-                                                         s.id, vp, sa, s.ctl, iv)
+  kForce<double, 0, M, kUDouble><<<grid, kBlock, 0, st>>>(s.G, o, t64, x, xn, s.type, s.df64, s.wmask, xp(s.v), \
+                                                         xp(s.f), s.id, vp, sa, s.ctl, iv)
 #define MISA_FORCE_MU(A, M, U)                                                                              \
   kForceM<A, M, U><<<grid, kBlock, 0, st>>>(s.G, o, t32, x, xn, s.type, s.rec[s.cur], s.rec[1 - s.cur], s.dl, \
                                             s.wmask, xp(s.v), xp(s.f), s.id, vp, sa, s.ctl, iv)

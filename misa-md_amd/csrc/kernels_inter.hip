@@ -846,7 +846,7 @@ void launchDecide(const DevState& s, int mode, bool mixed, hipStream_t st) {
   A.lrec = mixed ? s.rec[s.cur] : nullptr;
   A.in = s.in;
   A.ctl = s.ctl;
-  A.skin = mixed ? s.skin : 0.0;
+  A.skin = s.skin;
   kDecide<<<1, kDT, 0, st>>>(A);
 }
 

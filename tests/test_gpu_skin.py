@@ -1,4 +1,4 @@
-"""The neighbour skin of the mixed density pass (needs an MI355X): between
+"""The neighbour skin of the density pass (needs an MI355X): between
 full scans the pass visits only the offsets that were within cutoff + skin,
 which must leave every sum, and so every trajectory, bit-identical to a pass
 that scans all offsets every step: at equilibrium with a fixed step, and
@@ -16,7 +16,7 @@ F = 5.6 / 2.855
 FIELDS = ("id", "type", "x", "v", "f", "rho", "df")
 
 
-def _pair(setfl, cells, seed, tset, ratio):
+def _pair(setfl, cells, seed, tset, ratio, precision):
     sim = O.Sim(cells, A, F, setfl)
     sim.build_world(seed, tset, ratio, seed)
     atoms = sim.atoms().copy()
@@ -25,7 +25,7 @@ def _pair(setfl, cells, seed, tset, ratio):
         dev = M.EamDevice(cells)
         dev.make_offsets()
         dev.set_potential(setfl)
-        dev.set_precision(M.PREC_MIXED_KAHAN)
+        dev.set_precision(precision)
         dev.set_neighbour_skin(skin)
         dev.upload(atoms)
         out.append(dev)
@@ -43,8 +43,9 @@ def _same(a, b, what):
         np.testing.assert_array_equal(ia[k], ib[k], err_msg="%s: inter %s" % (what, k))
 
 
-def test_skin_at_equilibrium(setfl_fecuni):
-    a, b = _pair(setfl_fecuni, (8, 8, 12), 21, 900.0, (900, 70, 30))
+@pytest.mark.parametrize("precision", [M.PREC_MIXED_KAHAN, M.PREC_FP64])
+def test_skin_at_equilibrium(setfl_fecuni, precision):
+    a, b = _pair(setfl_fecuni, (8, 8, 12), 21, 900.0, (900, 70, 30), precision)
     for d in (a, b):
         d.set_timestep(0.001)
         d.compute_forces()
@@ -56,8 +57,9 @@ def test_skin_at_equilibrium(setfl_fecuni):
     b.close()
 
 
-def test_skin_through_a_cascade(setfl_fecuni):
-    a, b = _pair(setfl_fecuni, (10, 10, 10), 7, 600.0, (985, 10, 5))
+@pytest.mark.parametrize("precision", [M.PREC_MIXED_KAHAN, M.PREC_FP64])
+def test_skin_through_a_cascade(setfl_fecuni, precision):
+    a, b = _pair(setfl_fecuni, (10, 10, 10), 7, 600.0, (985, 10, 5), precision)
     for d in (a, b):
         d.set_timestep(0.0005)
         d.compute_forces()
