@@ -148,7 +148,7 @@ int misa_set_timestep(misa_handle* h, double dt);
  * inter atoms, computed on the device from the velocities at the start of the
  * step; max_disp = 0 returns to the fixed step of misa_set_timestep. */
 int misa_set_adaptive_dt(misa_handle* h, double dt_max, double max_disp);
-/* Neighbour skin of the mixed density pass (Angstrom; default 0.2, 0 turns
+/* Neighbour skin of the density pass, both precisions (Angstrom; default 0.2, 0 turns
  * it off; no counterpart in the reference, whose lists are rebuilt every
  * step).  A full scan of the pruned offsets also records, per wavefront, the
  * offsets within cutoff + skin; until the lattice atoms' summed per-step

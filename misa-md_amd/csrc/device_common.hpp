@@ -117,14 +117,16 @@ struct Tables {
   T rhomax;
   // packed records for the resident kernels
   const T* rhoV;   // [3][nr][4]: value cubic c3..c6 of rho_e
-  // [9][nr][8]: FP64: value cubics of (r phi)_eiej and rho_ej (c3..c6 each);
-  // FP32: value cubic of (r phi)_eiej, then the slope cubic of rho_ej in the
-  // knot coordinate already divided by the knot spacing (3 c3/dr, 2 c4/dr, c5/dr)
-  const T* pairF;
-  const T* rhoD;   // [3][nr][4]: slope cubic of rho_e as in pairF (FP32 force pass)
-  // [9][nr][4]: value cubic of (r phi)_eiej alone, 16 B a knot, for the
-  // mixed force pass's gathers (eight knots to a 128-byte cache line)
-  const T* pairZ;
+  const T* pairF;  // [9][nr][8]: value cubics of (r phi)_eiej and rho_ej (c3..c6 each)
+  // Compact knots of the mixed passes (FP32 only).  A cubic Hermite segment
+  // m is fixed by the values y and slopes s (per unit knot coordinate) at its
+  // two ends and one more coefficient, so a knot need only hold what segment
+  // m cannot take from knot m+1, and one load of 12 or 16 bytes at knot m
+  // reads a whole segment.  The narrower the knot, the fewer cache lines the
+  // gathers of a wavefront touch.
+  const T* rhoH;   // [3][nr][2]: {y, s} of rho_e (density pass)
+  const T* rhoS;   // [3][nr][2]: {s, 2 c4} of rho_e, times 1/dr (slopes in the force pass)
+  const T* pairH;  // [9][nr][3]: {s, y, c4} of (r phi)_eiej (force pass)
 };
 constexpr int kPairRec = 8;
 constexpr int kRhoRec = 4;

@@ -472,7 +472,7 @@ __global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<f
   if (!isVacant(ti)) {
     const float4 me = rec[s];
     const Rsrc rr = rsrcBelow(rec, O.lo);
-    const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoV));
+    const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoH));
     const uint32_t o16 = static_cast<uint32_t>(s) * 16u;
     const uint32_t* sh = O.sh[p];
     const float4* D = dl + p * kMaxLatOffsets;
@@ -560,8 +560,8 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
   const float4 me = rec[s];
   const float dfi = me.w;
   const Rsrc rr = rsrcBelow(rec, O.lo);
-  const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairZ));
-  const Rsrc rtabD = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoD));
+  const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairH));
+  const Rsrc rtabD = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoS));
   const uint32_t o16 = static_cast<uint32_t>(s) * 16u;
   const uint32_t pairBase = static_cast<uint32_t>(ei * 3) * static_cast<uint32_t>(tb.nr);
   const uint32_t* sh = O.sh[p];

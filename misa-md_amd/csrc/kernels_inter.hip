@@ -445,7 +445,7 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
     A.ctl->n_prev = ne;
     A.ctl->n_far_prev = nfar;
     A.ctl->n_far = 0;
-    // Neighbour skin of the mixed density pass.  Since the last full scan no
+    // Neighbour skin of the density pass (both precisions).  Since the last full scan no
     // lattice atom has moved farther than disp_acc, so no pair has come closer
     // by more than 2 disp_acc: an offset beyond cutoff + skin for every lane
     // then is beyond the cutoff now while disp_acc < skin / 2.  A refill puts

@@ -30,6 +30,19 @@ __device__ __forceinline__ float dist2M(const float4& me, const float4& nb, cons
   return __fmaf_rn(d2, d2, __fmaf_rn(d1, d1, __fmul_rn(d0, d0)));
 }
 
+// Value of the cubic Hermite segment between knots {y0, s0} and {y1, s1}
+// (slopes per unit knot coordinate) at local coordinate q: the segment's
+// coefficients c4 = 3 (y1 - y0) - 2 s0 - s1 and c3 = s0 + s1 - 2 (y1 - y0),
+// as the eam/alloy interpolation defines them.  y1 - y0 is exact for
+// neighbouring knots of like sign, so the value is as accurate as the knots.
+__device__ __forceinline__ float hermiteValue(const float4& k, float q) {
+#pragma clang fp contract(off)
+  const float dy = k.z - k.x;
+  const float c4 = fmaf(3.0f, dy, -fmaf(2.0f, k.y, k.w));
+  const float c3 = fmaf(-2.0f, dy, k.y + k.w);
+  return fmaf(fmaf(fmaf(c3, q, c4), q, k.y), q, k.x);
+}
+
 // rho_i += sum over the group of rho_{t_j}(r_ij) (pot eam::chargeDensity);
 // in[u] tells whether neighbour u is within the cutoff, sk[u] whether it is
 // within the cutoff plus the neighbour skin (r^2 < rcs2).
@@ -49,20 +62,22 @@ __device__ __forceinline__ void rhoGroupM(const float4& me, const float4 (&nb)[V
     if (in[u]) {
       const int m = knotM(r2 * __builtin_amdgcn_rsqf(r2), tb.rdr, tb.nr, pp[u]);
       const uint32_t row = __umul24(static_cast<uint32_t>(recSpecies(nb[u].x)), static_cast<uint32_t>(tb.nr));
-      cr[u] = ldRec4(rtab, (row + static_cast<uint32_t>(m)) * 16u, 0);
+      cr[u] = ldRec4(rtab, (row + static_cast<uint32_t>(m)) * 8u, 0);  // {y0, s0, y1, s1}
     }
   }
 #pragma unroll
   for (int u = 0; u < V; ++u)
-    if (in[u]) acc.add(fmaf(fmaf(fmaf(cr[u].x, pp[u], cr[u].y), pp[u], cr[u].z), pp[u], cr[u].w));
+    if (in[u]) acc.add(hermiteValue(cr[u], pp[u]));
 }
 
 // f_i += sum over the group of (x_i - x_j) fpair_ij (pot eam::toForce): the
-// value cubic of r*phi of the pair, the slope cubic of the neighbour's
-// density and, for unlike pairs, the slope cubic of the centre's density,
-// each from a table of 16-byte knots (the gathers of a wavefront spread over
-// a range of knots, so the narrower the knot, the fewer cache lines a gather
-// touches).  A slope cubic is already divided by the knot spacing.
+// value and slope of r*phi of the pair from its segment {s0, y0, c4, s1}
+// (12-byte knots), the slope of the neighbour's density and, for unlike
+// pairs, of the centre's density from their segments {s0, a, s1} (8-byte
+// knots, already divided by the knot spacing, a = 2 c4).  The gathers of a
+// wavefront spread over a range of knots, so the narrower the knot, the fewer
+// cache lines a gather touches.  With 3 c3 = s1 - s0 - 2 c4, the slope of a
+// segment is s0 + q (2 c4 + 3 c3 q).
 template <int ACC, int V>
 __device__ __forceinline__ void forceGroupM(const float4& me, float dfi, int ei, uint32_t pairBase,
                                             const float4 (&nb)[V], const float4 (&d)[V], float rc2, Rsrc rtab,
@@ -81,24 +96,26 @@ __device__ __forceinline__ void forceGroupM(const float4& me, float dfi, int ei,
       const int m = knotM(r2 * ri[u], tb.rdr, tb.nr, pp[u]);
       const uint32_t ej = static_cast<uint32_t>(recSpecies(nb[u].x));
       const uint32_t rowj = __umul24(ej, static_cast<uint32_t>(tb.nr)) + static_cast<uint32_t>(m);
-      zc[u] = ldRec4(rtab, (pairBase + rowj) * 16u, 0);
-      sj[u] = ld3(rtabD, rowj * 16u);
+      zc[u] = ldRec4(rtab, (pairBase + rowj) * 12u, 0);
+      sj[u] = ld3(rtabD, rowj * 8u);
       like[u] = ej == static_cast<uint32_t>(ei);
       si[u] = f32x3{0.0f, 0.0f, 0.0f};
       if (!like[u])  // rho_ei differs from rho_ej only for unlike pairs
         si[u] = ld3(rtabD, (__umul24(static_cast<uint32_t>(ei), static_cast<uint32_t>(tb.nr)) +
-                            static_cast<uint32_t>(m)) * 16u);
+                            static_cast<uint32_t>(m)) * 8u);
     }
   }
 #pragma unroll
   for (int u = 0; u < V; ++u) {
     if (in[u]) {
       const float q = pp[u], rinv = ri[u];
-      const float4 z = zc[u];
-      const float z2 = fmaf(fmaf(fmaf(z.x, q, z.y), q, z.z), q, z.w);
-      const float z2p = fmaf(fmaf(3.0f * z.x, q, 2.0f * z.y), q, z.z) * tb.rdr;
-      const float rhojp = fmaf(fmaf(sj[u].x, q, sj[u].y), q, sj[u].z);
-      const float rhoip = like[u] ? rhojp : fmaf(fmaf(si[u].x, q, si[u].y), q, si[u].z);
+      const float4 z = zc[u];  // s0, y0, c4, s1
+      const float c3x3 = fmaf(-2.0f, z.z, z.w - z.x);
+      const float z2 = fmaf(fmaf(fmaf(c3x3 * (1.0f / 3.0f), q, z.z), q, z.x), q, z.y);
+      const float z2p = fmaf(fmaf(c3x3, q, 2.0f * z.z), q, z.x) * tb.rdr;
+      const float rhojp = fmaf(fmaf((sj[u].z - sj[u].x) - sj[u].y, q, sj[u].y), q, sj[u].x);
+      const float rhoip =
+          like[u] ? rhojp : fmaf(fmaf((si[u].z - si[u].x) - si[u].y, q, si[u].y), q, si[u].x);
       const float phi = z2 * rinv;
       const float phip = (z2p - phi) * rinv;
       const float psip = fmaf(dfi, rhojp, fmaf(nb[u].w, rhoip, phip));
