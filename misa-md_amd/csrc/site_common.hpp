@@ -216,6 +216,7 @@ template <>
 struct Sum1<1> {
   float s = 0, c = 0;
   __device__ __forceinline__ void add(float v) {
+#pragma clang fp contract(off)  // the compensation must see v rounded, in every instantiation
     const float y = v - c;
     const float t = s + y;
     c = (t - s) - y;
@@ -250,6 +251,7 @@ struct Sum3<1> {
   Sum1<1> a, b, c;
   double f0 = 0, f1 = 0, f2 = 0;
   __device__ __forceinline__ void add(double d0, double d1, double d2, float fp) {
+#pragma clang fp contract(off)
     a.add(static_cast<float>(d0) * fp);
     b.add(static_cast<float>(d1) * fp);
     c.add(static_cast<float>(d2) * fp);
