@@ -151,11 +151,13 @@ int buildPlaneOffsets(misa_handle* h) {
   return MISA_OK;
 }
 
-// Traversal of the site kernels: z-slab per XCD, y-tiles of up to 16 rows.
+// Traversal of the site kernels: z-slab per XCD, y-tiles of up to 32 rows
+// (on the 200^3 cascade box, tiles of 20-25 rows step about 1 % faster than
+// tiles of 10, and tiles of 40 rows or more are slower again).
 void makeWalk(Geom& G) {
   Walk& W = G.W;
   W.ty = 1;
-  for (int t = std::min(16, G.n1); t >= 1; t--)
+  for (int t = std::min(32, G.n1); t >= 1; t--)
     if (G.n1 % t == 0) {
       W.ty = t;
       break;
