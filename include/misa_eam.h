@@ -212,6 +212,16 @@ int misa_autotune(misa_handle* h, int64_t steps, int32_t* best_ty, double* best_
 /* Set / read the launch shape directly (y-tile rows, gathers in flight). */
 int misa_set_launch_shape(misa_handle* h, int32_t ty, int32_t gathers_rho, int32_t gathers_force);
 int misa_get_launch_shape(const misa_handle* h, int32_t* ty, int32_t* gathers_rho, int32_t* gathers_force);
+/* Mixed-precision force pass in persistent blocks that hold the pruned
+ * offsets and the majority species' spline knots in LDS: MISA_LDS_AUTO
+ * (default; boxes large enough that every block walks several virtual
+ * blocks), MISA_LDS_ON, MISA_LDS_OFF.  No counterpart in the reference (a
+ * tuning switch of this build's kernels); the pairs, their order and their
+ * rounding are the same either way, so results are bit-identical. */
+#define MISA_LDS_AUTO 0
+#define MISA_LDS_ON 1
+#define MISA_LDS_OFF 2
+int misa_set_lds_tables(misa_handle* h, int32_t mode);
 
 /* ---- host mirror of libcomm's domain at one rank ----
  *

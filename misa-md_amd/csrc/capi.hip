@@ -342,6 +342,10 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
   if (rc == MISA_OK) rc |= devAlloc(h, &s.type, slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.id, slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.ctl, 1);
+  if (rc == MISA_OK) {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess) s.ncu = ncu;
+  }
   const int64_t blocks = 16 * G.W.bpx;
   // room for the site kernels' 7 partials per block and the inter kernels' 5
   s.npartials = std::max<int64_t>(8 * blocks, 8 * kInterBlocks);
@@ -456,9 +460,9 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
   const int64_t oRhoV = 3 * sF + 9 * sR;            // [3][nr][4] density value cubics
   const int64_t oPair = oRhoV + 3 * nr * kRhoRec;  // [9][nr][8] force records per ordered pair
   const int64_t oRhoH = oPair + 9 * nr * kPairRec;  // [3][nr][2] compact density knots {y, s}
-  const int64_t oRhoS = oRhoH + 3 * nr * 2;         // [3][nr][2] compact density slope knots {s, 2 c4} / dr
-  const int64_t oPairH = oRhoS + 3 * nr * 2;        // [9][nr][3] compact r*phi knots {s, y, c4}
-  const int64_t total = oPairH + 9 * nr * 3;
+  const int64_t oRhoD = oRhoH + 3 * nr * 2;         // [3][nr][4] density slope cubics / dr (FP32 force pass)
+  const int64_t oPairZ = oRhoD + 3 * nr * kRhoRec;  // [9][nr][4] r*phi value cubics (FP32 force pass)
+  const int64_t total = oPairZ + 9 * nr * kRhoRec;
   std::vector<double> h64(static_cast<size_t>(total), 0.0);
   auto put = [&](int64_t base, const std::vector<double>& c7, int n) {
     for (int m = 0; m < n; m++) {
@@ -500,18 +504,20 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
       const std::vector<double>& Z = h->cZ[ea * (ea + 1) / 2 + eb];
       for (int64_t m = 0; m < nr; m++) {
         double* r = &h64[oPair + ((a * 3 + b2) * nr + m) * kPairRec];
+        double* z = &h64[oPairZ + ((a * 3 + b2) * nr + m) * kRhoRec];
         for (int q = 0; q < 4; q++) {
           r[q] = Z[m * 7 + 3 + q];
           r[4 + q] = Rb[m * 7 + 3 + q];
+          z[q] = Z[m * 7 + 3 + q];
         }
       }
     }
   std::vector<float> h32(h64.size());
   for (size_t i = 0; i < h64.size(); i++) h32[i] = static_cast<float>(h64[i]);
   const double rdr = 1.0 / sf.dr;
-  // compact FP32 knots (Hermite form of the same segments): c[5] is the knot
-  // slope s, c[6] the value y, c[4] the quadratic coefficient of the segment
-  // that starts at the knot
+  // FP32 density knots {y, s} of the density pass (Hermite form of the same
+  // segments: c[5] is the knot slope s, c[6] the value y), and the density
+  // slope cubics of the force pass divided by the knot spacing
   for (int t = 0; t < 3; t++) {
     if (el[t] < 0) continue;
     const std::vector<double>& R = h->cR[el[t]];
@@ -519,23 +525,13 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
       float* kh = &h32[oRhoH + (t * nr + m) * 2];
       kh[0] = static_cast<float>(R[m * 7 + 6]);
       kh[1] = static_cast<float>(R[m * 7 + 5]);
-      float* ks = &h32[oRhoS + (t * nr + m) * 2];
-      ks[0] = static_cast<float>(R[m * 7 + 5] * rdr);
-      ks[1] = static_cast<float>(2.0 * R[m * 7 + 4] * rdr);
+      float* kd = &h32[oRhoD + (t * nr + m) * kRhoRec];
+      kd[0] = static_cast<float>(3.0 * R[m * 7 + 3] * rdr);
+      kd[1] = static_cast<float>(2.0 * R[m * 7 + 4] * rdr);
+      kd[2] = static_cast<float>(R[m * 7 + 5] * rdr);
+      kd[3] = 0.0f;
     }
   }
-  for (int a = 0; a < 3; a++)
-    for (int b2 = 0; b2 < 3; b2++) {
-      if (el[a] < 0 || el[b2] < 0) continue;
-      const int ea = std::max(el[a], el[b2]), eb = std::min(el[a], el[b2]);
-      const std::vector<double>& Z = h->cZ[ea * (ea + 1) / 2 + eb];
-      for (int64_t m = 0; m < nr; m++) {
-        float* kz = &h32[oPairH + ((a * 3 + b2) * nr + m) * 3];
-        kz[0] = static_cast<float>(Z[m * 7 + 5]);
-        kz[1] = static_cast<float>(Z[m * 7 + 6]);
-        kz[2] = static_cast<float>(Z[m * 7 + 4]);
-      }
-    }
   if (!h->tab64) {
     if (devAlloc(h, &h->tab64, static_cast<size_t>(total)) != MISA_OK) return MISA_E_HIP;
     if (devAlloc(h, &h->tab32, static_cast<size_t>(total)) != MISA_OK) return MISA_E_HIP;
@@ -557,7 +553,7 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
   a64.rhomax = (sf.nrho - 1) * sf.drho;
   a64.rhoV = h->tab64 + oRhoV;
   a64.pairF = h->tab64 + oPair;
-  a64.rhoH = a64.rhoS = a64.pairH = nullptr;  // FP32 passes only
+  a64.rhoH = a64.rhoD = a64.pairZ = nullptr;  // FP32 passes only
   Tables<float>& a32 = h->t32;
   a32.F = h->tab32;
   a32.rho = h->tab32 + 3 * sF;
@@ -572,8 +568,8 @@ int misa_set_potential_file(misa_handle* h, const char* path) {
   a32.rhoV = h->tab32 + oRhoV;
   a32.pairF = h->tab32 + oPair;
   a32.rhoH = h->tab32 + oRhoH;
-  a32.rhoS = h->tab32 + oRhoS;
-  a32.pairH = h->tab32 + oPairH;
+  a32.rhoD = h->tab32 + oRhoD;
+  a32.pairZ = h->tab32 + oPairZ;
   h->have_pot = true;
   return MISA_OK;
 }
@@ -1304,6 +1300,16 @@ int misa_set_launch_shape(misa_handle* h, int32_t ty, int32_t gathers_rho, int32
   h->s.G.W.ty = ty;
   h->s.uRho = gathers_rho;
   h->s.uForce = gathers_force;
+  h->gen++;
+  return MISA_OK;
+}
+
+static_assert(MISA_LDS_AUTO == kLdsAuto && MISA_LDS_ON == kLdsOn && MISA_LDS_OFF == kLdsOff, "LDS modes");
+int misa_set_lds_tables(misa_handle* h, int32_t mode) {
+  if (!h) return MISA_E_ARG;
+  if (mode != MISA_LDS_AUTO && mode != MISA_LDS_ON && mode != MISA_LDS_OFF)
+    return fail(h, MISA_E_ARG, "LDS table mode must be MISA_LDS_AUTO, MISA_LDS_ON or MISA_LDS_OFF");
+  h->s.lds = mode;
   h->gen++;
   return MISA_OK;
 }

@@ -125,8 +125,10 @@ struct Tables {
   // reads a whole segment.  The narrower the knot, the fewer cache lines the
   // gathers of a wavefront touch.
   const T* rhoH;   // [3][nr][2]: {y, s} of rho_e (density pass)
-  const T* rhoS;   // [3][nr][2]: {s, 2 c4} of rho_e, times 1/dr (slopes in the force pass)
-  const T* pairH;  // [9][nr][3]: {s, y, c4} of (r phi)_eiej (force pass)
+  // FP32 force pass (global memory and LDS alike): 16-byte knots that need
+  // no coefficient arithmetic and sit 16-byte aligned for LDS reads
+  const T* rhoD;   // [3][nr][4]: slope cubic of rho_e divided by dr {3 c3, 2 c4, c5, 0} / dr
+  const T* pairZ;  // [9][nr][4]: value cubic of (r phi)_eiej {c3, c4, c5, c6}
 };
 constexpr int kPairRec = 8;
 constexpr int kRhoRec = 4;

@@ -560,8 +560,8 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
   const float4 me = rec[s];
   const float dfi = me.w;
   const Rsrc rr = rsrcBelow(rec, O.lo);
-  const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairH));
-  const Rsrc rtabD = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoS));
+  const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairZ));
+  const Rsrc rtabD = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoD));
   const uint32_t o16 = static_cast<uint32_t>(s) * 16u;
   const uint32_t pairBase = static_cast<uint32_t>(ei * 3) * static_cast<uint32_t>(tb.nr);
   const uint32_t* sh = O.sh[p];
@@ -589,6 +589,112 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
   }();
   if (MODE == 0 && vp.vmax) waveMaxTo(vb, vp.vmax);
   if (MODE == 1) waveMaxTo(db, &ctl->dispmax2);
+}
+
+// The mixed force pass in persistent blocks, one per compute unit, that hold
+// in LDS the lattice vector and record shift of every pruned offset (read as
+// one broadcast per offset), the Fe-Fe r*phi cubics and the Fe density slope
+// cubics from knot mlo on (one block fills them once).  Per site the same
+// pairs, in the same order and with the same rounding, as kForceM.
+template <int ACC, int MODE, int U>
+__global__ __launch_bounds__(kBlockP) void kForceL(Geom G, LatOffsets O, Tables<float> tb, CXPtr x, XPtr xn,
+                                                   const int8_t* __restrict__ type, const float4* __restrict__ rec,
+                                                   float4* __restrict__ recn, const float4* __restrict__ dl,
+                                                   const uint64_t* __restrict__ wmask, XPtr v, XPtr f,
+                                                   const uint64_t* __restrict__ id, VerletParams vp, StageArgs st,
+                                                   Ctl* ctl, InterView iv, int mlo, int nkn) {
+  __shared__ float4 offT[2 * kMaxLatOffsets];  // {lattice vector, record shift * 16}
+  extern __shared__ float4 knT[];              // [nkn] Fe-Fe r*phi cubics, then [nkn] Fe slope cubics
+  for (int i = threadIdx.x; i < 2 * kMaxLatOffsets; i += kBlockP) {
+    const int q = i / kMaxLatOffsets, k = i - q * kMaxLatOffsets;
+    float4 d = dl[i];
+    d.w = __uint_as_float(k < O.n[q] ? O.sh[q][k] * 16u : 0u);
+    offT[i] = d;
+  }
+  const float4* z4 = reinterpret_cast<const float4*>(tb.pairZ);  // row (Fe, Fe) is the first
+  const float4* s4 = reinterpret_cast<const float4*>(tb.rhoD);   // row Fe is the first
+  for (int i = threadIdx.x; i < nkn; i += kBlockP) {
+    knT[i] = z4[mlo + i];
+    knT[nkn + i] = s4[mlo + i];
+  }
+  __syncthreads();
+  const float4* zT = knT;
+  const float4* sT = knT + nkn;
+  const Rsrc rr = rsrcBelow(rec, O.lo);
+  const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairZ));
+  const Rsrc rtabD = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoD));
+  const float rc2 = static_cast<float>(G.rc2);
+  unsigned long long vb = 0;  // |v|^2 bits after the secondstep (MODE 0), largest over the lane's sites
+  unsigned long long db = 0;  // |dx|^2 bits of the drift (MODE 1), likewise
+  forVirtualBlocks(G, [&](uint32_t b, uint32_t tl) {
+    int64_t s;
+    int p, cx, cy, cz;
+    if (!siteOfBlock(G, b, tl, s, p, cx, cy, cz)) return;
+    const int8_t ti = type[s];
+    if (isVacant(ti)) {  // vacancy: keeps its stale position (atom.cpp:43-47)
+      vacancyForceM<MODE>(G, s, p, cx, cy, cz, x, xn, recn, f);
+      return;
+    }
+    const int ei = species(ti);
+    const float4 me = rec[s];
+    const float dfi = me.w;
+    const uint32_t o16 = static_cast<uint32_t>(s) * 16u;
+    const uint32_t pairBase = static_cast<uint32_t>(ei * 3) * static_cast<uint32_t>(tb.nr);
+    const float4* T = offT + p * kMaxLatOffsets;
+    Sum3<ACC> acc;
+    const uint32_t wv = b * (kBlock / 64) + __builtin_amdgcn_readfirstlane(tl >> 6);
+    auto group = [&](auto UC, const int* ks) {
+      constexpr int V = decltype(UC)::value;
+      float4 nb[V], d[V];
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        d[u] = T[ks[u]];
+        nb[u] = ldRec4(rr, o16 + __float_as_uint(d[u].w), 0);
+      }
+      forceGroupL<ACC, V>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, zT, sT, mlo, acc);
+    };
+    forMaskGroups<U>(uniform64(wmask[2 * wv]), uniform64(wmask[2 * wv + 1]), [&](const int* ks, int nv) {
+      if (nv == U) group(std::integral_constant<int, U>{}, ks);
+      else group(std::integral_constant<int, 1>{}, ks);
+    });
+    unsigned long long vbs = 0, dbs = 0;
+    forceTailM<ACC, MODE>(G, s, p, cx, cy, cz, ti, ei, dfi, acc, x, xn, type, rec, recn, v, f, id, vp, st, ctl, tb,
+                          iv, vbs, dbs);
+    vb = vbs > vb ? vbs : vb;
+    db = dbs > db ? dbs : db;
+  });
+  if (MODE == 0 && vp.vmax) waveMaxTo(vb, vp.vmax);
+  if (MODE == 1) waveMaxTo(db, &ctl->dispmax2);
+}
+
+// LDS knots of the force pass: the Fe-Fe r*phi and Fe slope cubics (32 bytes
+// a knot) from knot mlo to the last, within one block per compute unit
+// (160 KB of LDS) beside the offset table.
+constexpr int kForceLdsBytes = 160 * 1024 - 2 * kMaxLatOffsets * 16;
+struct LdsKnots {
+  int mlo, nkn;
+  size_t bytes;
+};
+LdsKnots ldsKnotsForce(int nr) {
+  LdsKnots k;
+  k.nkn = std::min(nr, kForceLdsBytes / 32);
+  k.mlo = nr - k.nkn;
+  k.bytes = static_cast<size_t>(32) * static_cast<size_t>(k.nkn);
+  return k;
+}
+bool setLdsBytes(const void* fn, int bytes) {
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess;
+}
+// One persistent block per compute unit, a whole number per XCD.
+unsigned persistentGrid(const DevState& s) { return static_cast<unsigned>(s.ncu / 8 * 8); }
+// The LDS-table force pass: on request, or by default when every persistent
+// block walks at least four virtual blocks per sub-block (a block's fill of
+// its tables then costs little beside its share of the pass).
+bool useLds(const DevState& s) {
+  if (s.ncu < 8 || s.lds == kLdsOff) return false;
+  if (s.lds == kLdsOn) return true;
+  const int64_t perXcd = static_cast<int64_t>(persistentGrid(s) / 8) * (kBlockP / kBlock);
+  return 2 * s.G.W.bpx >= 4 * perXcd;
 }
 
 constexpr int kUDouble = 2;
@@ -632,12 +738,24 @@ void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t
   const XPtr xn = xp(s.x[1 - s.cur]);
   const StageArgs sa = stageOf(s);
   const InterView iv = interView(s, mixed);
+  const bool lds = mixed && useLds(s);
+  const LdsKnots lk = ldsKnotsForce(t32.nr);
 #define MISA_FORCE(M) \
   kForce<double, 0, M, kUDouble><<<grid, kBlock, 0, st>>>(s.G, o, t64, x, xn, s.type, s.df64, s.wmask, xp(s.v), \
                                                          xp(s.f), s.id, vp, sa, s.ctl, iv)
-#define MISA_FORCE_MU(A, M, U)                                                                              \
-  kForceM<A, M, U><<<grid, kBlock, 0, st>>>(s.G, o, t32, x, xn, s.type, s.rec[s.cur], s.rec[1 - s.cur], s.dl, \
-                                            s.wmask, xp(s.v), xp(s.f), s.id, vp, sa, s.ctl, iv)
+#define MISA_FORCE_MU(A, M, U)                                                                                \
+  do {                                                                                                        \
+    if (lds) {                                                                                                \
+      static const bool attr = setLdsBytes(reinterpret_cast<const void*>(&kForceL<A, M, U>), kForceLdsBytes); \
+      (void)attr;                                                                                             \
+      kForceL<A, M, U><<<persistentGrid(s), kBlockP, lk.bytes, st>>>(                                         \
+          s.G, o, t32, x, xn, s.type, s.rec[s.cur], s.rec[1 - s.cur], s.dl, s.wmask, xp(s.v), xp(s.f), s.id, vp, \
+          sa, s.ctl, iv, lk.mlo, lk.nkn);                                                                     \
+    } else {                                                                                                  \
+      kForceM<A, M, U><<<grid, kBlock, 0, st>>>(s.G, o, t32, x, xn, s.type, s.rec[s.cur], s.rec[1 - s.cur],    \
+                                                s.dl, s.wmask, xp(s.v), xp(s.f), s.id, vp, sa, s.ctl, iv);    \
+    }                                                                                                         \
+  } while (0)
 #define MISA_FORCE_M(A, M)            \
   do {                                \
     if (s.uForce == 2)                \

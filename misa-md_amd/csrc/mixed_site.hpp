@@ -70,23 +70,35 @@ __device__ __forceinline__ void rhoGroupM(const float4& me, const float4 (&nb)[V
     if (in[u]) acc.add(hermiteValue(cr[u], pp[u]));
 }
 
-// f_i += sum over the group of (x_i - x_j) fpair_ij (pot eam::toForce): the
-// value and slope of r*phi of the pair from its segment {s0, y0, c4, s1}
-// (12-byte knots), the slope of the neighbour's density and, for unlike
-// pairs, of the centre's density from their segments {s0, a, s1} (8-byte
-// knots, already divided by the knot spacing, a = 2 c4).  The gathers of a
-// wavefront spread over a range of knots, so the narrower the knot, the fewer
-// cache lines a gather touches.  With 3 c3 = s1 - s0 - 2 c4, the slope of a
-// segment is s0 + q (2 c4 + 3 c3 q).
+// fpair of one pair (pot eam::toForce) from its segments: the value cubic
+// of r*phi {c3, c4, c5, c6}, the slope cubic of the neighbour's density and,
+// for an unlike pair, of the centre's density, each already divided by the
+// knot spacing.  Shared by the force passes that gather the segments from
+// global memory and from LDS, so that both round alike.
+__device__ __forceinline__ float pairFpM(float q, float rinv, const float4& z, const f32x3& sj, const f32x3& si,
+                                         bool like, float dfi, float dfj, float rdr) {
+#pragma clang fp contract(off)  // explicit FMAs only: every instantiation rounds alike
+  const float z2 = fmaf(fmaf(fmaf(z.x, q, z.y), q, z.z), q, z.w);
+  const float z2p = fmaf(fmaf(3.0f * z.x, q, 2.0f * z.y), q, z.z) * rdr;
+  const float rhojp = fmaf(fmaf(sj.x, q, sj.y), q, sj.z);
+  const float rhoip = like ? rhojp : fmaf(fmaf(si.x, q, si.y), q, si.z);
+  const float phi = z2 * rinv;
+  const float phip = (z2p - phi) * rinv;
+  const float psip = fmaf(dfi, rhojp, fmaf(dfj, rhoip, phip));
+  return -psip * rinv;
+}
+
+// f_i += sum over the group of (x_i - x_j) fpair_ij, the segments gathered
+// from the tables in global memory (16-byte knots).
 template <int ACC, int V>
 __device__ __forceinline__ void forceGroupM(const float4& me, float dfi, int ei, uint32_t pairBase,
                                             const float4 (&nb)[V], const float4 (&d)[V], float rc2, Rsrc rtab,
                                             Rsrc rtabD, const Tables<float>& tb, Sum3<ACC>& acc) {
-#pragma clang fp contract(off)  // explicit FMAs only: every instantiation rounds alike
+#pragma clang fp contract(off)
   bool in[V], like[V];
   float d0[V], d1[V], d2[V], pp[V], ri[V];
-  float4 zc[V];        // r*phi value cubic c3..c6
-  f32x3 sj[V], si[V];  // density slope cubics: neighbour's, and the centre's for unlike pairs
+  float4 zc[V];
+  f32x3 sj[V], si[V];
 #pragma unroll
   for (int u = 0; u < V; ++u) {
     const float r2 = dist2M(me, nb[u], d[u], d0[u], d1[u], d2[u]);
@@ -96,32 +108,72 @@ __device__ __forceinline__ void forceGroupM(const float4& me, float dfi, int ei,
       const int m = knotM(r2 * ri[u], tb.rdr, tb.nr, pp[u]);
       const uint32_t ej = static_cast<uint32_t>(recSpecies(nb[u].x));
       const uint32_t rowj = __umul24(ej, static_cast<uint32_t>(tb.nr)) + static_cast<uint32_t>(m);
-      zc[u] = ldRec4(rtab, (pairBase + rowj) * 12u, 0);
-      sj[u] = ld3(rtabD, rowj * 8u);
+      zc[u] = ldRec4(rtab, (pairBase + rowj) * 16u, 0);
+      sj[u] = ld3(rtabD, rowj * 16u);
       like[u] = ej == static_cast<uint32_t>(ei);
       si[u] = f32x3{0.0f, 0.0f, 0.0f};
       if (!like[u])  // rho_ei differs from rho_ej only for unlike pairs
         si[u] = ld3(rtabD, (__umul24(static_cast<uint32_t>(ei), static_cast<uint32_t>(tb.nr)) +
-                            static_cast<uint32_t>(m)) * 8u);
+                            static_cast<uint32_t>(m)) * 16u);
     }
   }
 #pragma unroll
+  for (int u = 0; u < V; ++u)
+    if (in[u]) acc.add(d0[u], d1[u], d2[u], pairFpM(pp[u], ri[u], zc[u], sj[u], si[u], like[u], dfi, nb[u].w, tb.rdr));
+}
+
+// forceGroupM with the Fe-Fe r*phi cubics (zT) and the Fe density slope
+// cubics (sT) from knot mlo on in LDS: every lane reads its segments there,
+// at a clamped knot; a lane whose pair involves another species, or lies
+// below knot mlo, then re-reads what it needs from global memory, in a
+// branch that only a wavefront with such a lane enters (after the LDS reads,
+// so that no global load waits in front of them).
+template <int ACC, int V>
+__device__ __forceinline__ void forceGroupL(const float4& me, float dfi, int ei, uint32_t pairBase,
+                                            const float4 (&nb)[V], const float4 (&d)[V], float rc2, Rsrc rtab,
+                                            Rsrc rtabD, const Tables<float>& tb, const float4* zT,
+                                            const float4* sT, int mlo, Sum3<ACC>& acc) {
+#pragma clang fp contract(off)
+  bool in[V], like[V], fz[V], fj[V], fi[V];
+  float d0[V], d1[V], d2[V], pp[V], ri[V];
+  int mm[V];
+  uint32_t ejv[V];
+  float4 zc[V];
+  f32x3 sj[V], si[V];
+  bool any = false;
+#pragma unroll
   for (int u = 0; u < V; ++u) {
-    if (in[u]) {
-      const float q = pp[u], rinv = ri[u];
-      const float4 z = zc[u];  // s0, y0, c4, s1
-      const float c3x3 = fmaf(-2.0f, z.z, z.w - z.x);
-      const float z2 = fmaf(fmaf(fmaf(c3x3 * (1.0f / 3.0f), q, z.z), q, z.x), q, z.y);
-      const float z2p = fmaf(fmaf(c3x3, q, 2.0f * z.z), q, z.x) * tb.rdr;
-      const float rhojp = fmaf(fmaf((sj[u].z - sj[u].x) - sj[u].y, q, sj[u].y), q, sj[u].x);
-      const float rhoip =
-          like[u] ? rhojp : fmaf(fmaf((si[u].z - si[u].x) - si[u].y, q, si[u].y), q, si[u].x);
-      const float phi = z2 * rinv;
-      const float phip = (z2p - phi) * rinv;
-      const float psip = fmaf(dfi, rhojp, fmaf(nb[u].w, rhoip, phip));
-      acc.add(d0[u], d1[u], d2[u], -psip * rinv);
+    const float r2 = dist2M(me, nb[u], d[u], d0[u], d1[u], d2[u]);
+    in[u] = r2 < rc2;
+    ri[u] = __builtin_amdgcn_rsqf(r2);
+    mm[u] = knotM(r2 * ri[u], tb.rdr, tb.nr, pp[u]);
+    ejv[u] = static_cast<uint32_t>(recSpecies(nb[u].x));
+    like[u] = ejv[u] == static_cast<uint32_t>(ei);
+    const int j = max(mm[u] - mlo, 0);
+    zc[u] = zT[j];
+    const float4 s4 = sT[j];
+    sj[u] = f32x3{s4.x, s4.y, s4.z};
+    si[u] = sj[u];
+    const bool low = mm[u] < mlo;
+    fz[u] = in[u] && ((static_cast<uint32_t>(ei) | ejv[u]) != 0u || low);
+    fj[u] = in[u] && (ejv[u] != 0u || low);
+    fi[u] = in[u] && !like[u] && (ei != 0 || low);
+    any |= fz[u];
+  }
+  if (__ballot(any)) {
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      const uint32_t rowj = __umul24(ejv[u], static_cast<uint32_t>(tb.nr)) + static_cast<uint32_t>(mm[u]);
+      if (fz[u]) zc[u] = ldRec4(rtab, (pairBase + rowj) * 16u, 0);
+      if (fj[u]) sj[u] = ld3(rtabD, rowj * 16u);
+      if (fi[u])
+        si[u] = ld3(rtabD, (__umul24(static_cast<uint32_t>(ei), static_cast<uint32_t>(tb.nr)) +
+                            static_cast<uint32_t>(mm[u])) * 16u);
     }
   }
+#pragma unroll
+  for (int u = 0; u < V; ++u)
+    if (in[u]) acc.add(d0[u], d1[u], d2[u], pairFpM(pp[u], ri[u], zc[u], sj[u], si[u], like[u], dfi, nb[u].w, tb.rdr));
 }
 
 // After the lattice sum of site s: the inter atoms and, near a far lattice

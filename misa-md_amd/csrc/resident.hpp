@@ -54,6 +54,9 @@ struct InterDev {
   FullPlaneOffsets* full = nullptr;
 };
 
+// Choice of the LDS-table force pass (the values of MISA_LDS_AUTO / ON / OFF).
+constexpr int kLdsAuto = 0, kLdsOn = 1, kLdsOff = 2;
+
 struct DevState {
   Geom G{};
   double* x[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
@@ -73,6 +76,10 @@ struct DevState {
   uint64_t* wmask = nullptr;
   // gathers in flight per wave in the mixed passes (chosen by misa_autotune)
   int uRho = 4, uForce = 4;
+  // compute units of the device (persistent launches of the LDS-table force
+  // pass) and the choice of that pass (MISA_LDS_AUTO / ON / OFF)
+  int ncu = 0;
+  int lds = 0;  // kLdsAuto, kLdsOn, kLdsOff
   // neighbour skin of the mixed density pass (A; 0: every step scans all
   // pruned offsets) and, per wavefront, the pruned offsets within cutoff +
   // skin for some lane at the last full scan (2 words)

@@ -27,13 +27,15 @@ __device__ __forceinline__ int64_t slotOf(const Geom& G, int p, int cx, int cy, 
 // dispatcher deals workgroups to the XCDs round-robin); the k-th block of XCD
 // c takes parity k & 1 and local block k >> 1 of that XCD's z-slab, whose
 // sites are ordered (y-tile, z, y within tile, x).
-__device__ __forceinline__ bool siteOfThread(const Geom& G, int64_t& s, int& p, int& cx, int& cy, int& cz) {
-  const uint32_t b = blockIdx.x;
+// siteOfBlock: the same for thread tl of block b (also a virtual block of a
+// persistent launch, whose XCD is b % 8 as well).
+__device__ __forceinline__ bool siteOfBlock(const Geom& G, uint32_t b, uint32_t tl, int64_t& s, int& p, int& cx,
+                                            int& cy, int& cz) {
   const int c = static_cast<int>(b & 7);
   const uint32_t k = b >> 3;
   p = static_cast<int>(k & 1);
   const int z0 = G.W.zlo[c], nzc = G.W.zlo[c + 1] - z0;
-  const uint32_t t = (k >> 1) * kBlock + threadIdx.x;
+  const uint32_t t = (k >> 1) * kBlock + tl;
   const uint32_t n0 = static_cast<uint32_t>(G.n0), ty = static_cast<uint32_t>(G.W.ty);
   if (nzc <= 0 || t >= static_cast<uint32_t>(nzc) * static_cast<uint32_t>(G.n1) * n0) return false;
   cx = static_cast<int>(t % n0);
@@ -44,6 +46,26 @@ __device__ __forceinline__ bool siteOfThread(const Geom& G, int64_t& s, int& p, 
   cy = static_cast<int>(r / static_cast<uint32_t>(nzc)) * G.W.ty + yy;
   s = slotOf(G, p, cx, cy, cz);
   return true;
+}
+__device__ __forceinline__ bool siteOfThread(const Geom& G, int64_t& s, int& p, int& cx, int& cy, int& cz) {
+  return siteOfBlock(G, blockIdx.x, threadIdx.x, s, p, cx, cy, cz);
+}
+
+// Persistent launches of the passes that hold tables in LDS: blocks of
+// kBlockP threads, a whole number of blocks per XCD (the grid is a multiple
+// of 8), each walking virtual blocks of kBlock threads of its own XCD.  A
+// virtual block covers the sites, and its wavefronts carry the ids, of the
+// block of the same index in a launch of siteGrid blocks, so per-wavefront
+// data (the wave masks) carries over between the two kinds of launch.
+constexpr int kBlockP = 1024;
+template <typename Body>
+__device__ __forceinline__ void forVirtualBlocks(const Geom& G, Body body) {
+  constexpr uint32_t V = kBlockP / kBlock;
+  const uint32_t c = blockIdx.x & 7u, q = blockIdx.x >> 3, Q = gridDim.x >> 3;
+  const uint32_t sub = __builtin_amdgcn_readfirstlane(threadIdx.x / kBlock);
+  const uint32_t tl = threadIdx.x % kBlock;
+  const uint32_t nvb = 2u * static_cast<uint32_t>(G.W.bpx);
+  for (uint32_t k = q * V + sub; k < nvb; k += Q * V) body(c + 8u * k, tl);
 }
 
 inline unsigned siteGrid(const Geom& G) { return static_cast<unsigned>(16 * G.W.bpx); }

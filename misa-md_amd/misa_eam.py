@@ -24,6 +24,9 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_MIXED_KAHAN = 17
 PREC_MIXED_F32SUM = 33
+LDS_AUTO = 0
+LDS_ON = 1
+LDS_OFF = 2
 
 A_FE = 2.855
 RC_FACTOR = 5.6 / 2.855
@@ -133,6 +136,7 @@ _SIGS = {
     "misa_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
                                      ctypes.POINTER(ctypes.c_double)]),
     "misa_set_launch_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "misa_set_lds_tables": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "misa_get_launch_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
                                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "misa_sim_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
@@ -315,6 +319,10 @@ class EamDevice:
     def set_launch_shape(self, ty, gathers_rho, gathers_force):
         self._check(self.lib.misa_set_launch_shape(self.h, int(ty), int(gathers_rho), int(gathers_force)),
                     "misa_set_launch_shape")
+
+    def set_lds_tables(self, mode):
+        """LDS_AUTO, LDS_ON or LDS_OFF: the mixed force pass with its majority-species tables in LDS."""
+        self._check(self.lib.misa_set_lds_tables(self.h, int(mode)), "misa_set_lds_tables")
 
     def launch_shape(self):
         a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
