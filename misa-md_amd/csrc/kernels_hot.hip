@@ -669,8 +669,9 @@ __global__ __launch_bounds__(kBlockP) void kForceL(Geom G, LatOffsets O, Tables<
 
 // LDS knots of the force pass: the Fe-Fe r*phi and Fe slope cubics (32 bytes
 // a knot) from knot mlo to the last, within one block per compute unit
-// (160 KB of LDS) beside the offset table.
-constexpr int kForceLdsBytes = 160 * 1024 - 2 * kMaxLatOffsets * 16;
+// (160 KB of LDS) beside the kernel's static LDS (the offset table, 3.5 KB;
+// 64 bytes are kept beyond it).
+constexpr int kForceLdsBytes = 160 * 1024 - 2 * kMaxLatOffsets * 16 - 64;
 struct LdsKnots {
   int mlo, nkn;
   size_t bytes;
@@ -745,9 +746,9 @@ void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t
                                                          xp(s.f), s.id, vp, sa, s.ctl, iv)
 #define MISA_FORCE_MU(A, M, U)                                                                                \
   do {                                                                                                        \
-    if (lds) {                                                                                                \
-      static const bool attr = setLdsBytes(reinterpret_cast<const void*>(&kForceL<A, M, U>), kForceLdsBytes); \
-      (void)attr;                                                                                             \
+    static const bool ldsReady =                                                                              \
+        setLdsBytes(reinterpret_cast<const void*>(&kForceL<A, M, U>), kForceLdsBytes);                       \
+    if (lds && ldsReady) {                                                                                    \
       kForceL<A, M, U><<<persistentGrid(s), kBlockP, lk.bytes, st>>>(                                         \
           s.G, o, t32, x, xn, s.type, s.rec[s.cur], s.rec[1 - s.cur], s.dl, s.wmask, xp(s.v), xp(s.f), s.id, vp, \
           sa, s.ctl, iv, lk.mlo, lk.nkn);                                                                     \
