@@ -134,7 +134,7 @@ __device__ __forceinline__ void forceGroupL(const float4& me, float dfi, int ei,
                                             Rsrc rtabD, const Tables<float>& tb, const float4* zT,
                                             const float4* sT, int mlo, Sum3<ACC>& acc) {
 #pragma clang fp contract(off)
-  bool in[V], like[V], fz[V], fj[V], fi[V];
+  bool in[V], like[V], fz[V];
   float d0[V], d1[V], d2[V], pp[V], ri[V];
   int mm[V];
   uint32_t ejv[V];
@@ -154,19 +154,18 @@ __device__ __forceinline__ void forceGroupL(const float4& me, float dfi, int ei,
     const float4 s4 = sT[j];
     sj[u] = f32x3{s4.x, s4.y, s4.z};
     si[u] = sj[u];
-    const bool low = mm[u] < mlo;
-    fz[u] = in[u] && ((static_cast<uint32_t>(ei) | ejv[u]) != 0u || low);
-    fj[u] = in[u] && (ejv[u] != 0u || low);
-    fi[u] = in[u] && !like[u] && (ei != 0 || low);
+    fz[u] = in[u] && ((static_cast<uint32_t>(ei) | ejv[u]) != 0u || mm[u] < mlo);
     any |= fz[u];
   }
-  if (__ballot(any)) {
+  if (__ballot(any)) {  // the pair needs r*phi from memory; the slopes as their species or knot say
 #pragma unroll
     for (int u = 0; u < V; ++u) {
+      if (!fz[u]) continue;
+      const bool low = mm[u] < mlo;
       const uint32_t rowj = __umul24(ejv[u], static_cast<uint32_t>(tb.nr)) + static_cast<uint32_t>(mm[u]);
-      if (fz[u]) zc[u] = ldRec4(rtab, (pairBase + rowj) * 16u, 0);
-      if (fj[u]) sj[u] = ld3(rtabD, rowj * 16u);
-      if (fi[u])
+      zc[u] = ldRec4(rtab, (pairBase + rowj) * 16u, 0);
+      if (ejv[u] != 0u || low) sj[u] = ld3(rtabD, rowj * 16u);
+      if (!like[u] && (ei != 0 || low))
         si[u] = ld3(rtabD, (__umul24(static_cast<uint32_t>(ei), static_cast<uint32_t>(tb.nr)) +
                             static_cast<uint32_t>(mm[u])) * 16u);
     }
