@@ -578,12 +578,13 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
       nb[u] = ldRec4(rr, o16, sh[ks[u]] * 16u);
       d[u] = D[ks[u]];
     }
-    forceGroupM<ACC, V>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, acc);
+    forceGroupM<ACC, V>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, ks, acc);
   };
   forMaskGroups<U>(uniform64(wmask[2 * wv]), uniform64(wmask[2 * wv + 1]), [&](const int* ks, int nv) {
     if (nv == U) group(std::integral_constant<int, U>{}, ks);
     else group(std::integral_constant<int, 1>{}, ks);
   });
+  flushChunk3(acc);
   forceTailM<ACC, MODE>(G, s, p, cx, cy, cz, ti, ei, dfi, acc, x, xn, type, rec, recn, v, f, id, vp, st, ctl, tb,
                         iv, vb, db);
   }();
@@ -651,12 +652,13 @@ __global__ __launch_bounds__(kBlockP) void kForceL(Geom G, LatOffsets O, Tables<
         d[u] = T[ks[u]];
         nb[u] = ldRec4(rr, o16 + __float_as_uint(d[u].w), 0);
       }
-      forceGroupL<ACC, V>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, zT, sT, mlo, acc);
+      forceGroupL<ACC, V>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, zT, sT, mlo, ks, acc);
     };
     forMaskGroups<U>(uniform64(wmask[2 * wv]), uniform64(wmask[2 * wv + 1]), [&](const int* ks, int nv) {
       if (nv == U) group(std::integral_constant<int, U>{}, ks);
       else group(std::integral_constant<int, 1>{}, ks);
     });
+    flushChunk3(acc);
     unsigned long long vbs = 0, dbs = 0;
     forceTailM<ACC, MODE>(G, s, p, cx, cy, cz, ti, ei, dfi, acc, x, xn, type, rec, recn, v, f, id, vp, st, ctl, tb,
                           iv, vbs, dbs);

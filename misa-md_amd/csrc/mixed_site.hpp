@@ -93,7 +93,7 @@ __device__ __forceinline__ float pairFpM(float q, float rinv, const float4& z, c
 template <int ACC, int V>
 __device__ __forceinline__ void forceGroupM(const float4& me, float dfi, int ei, uint32_t pairBase,
                                             const float4 (&nb)[V], const float4 (&d)[V], float rc2, Rsrc rtab,
-                                            Rsrc rtabD, const Tables<float>& tb, Sum3<ACC>& acc) {
+                                            Rsrc rtabD, const Tables<float>& tb, const int* ks, Sum3<ACC>& acc) {
 #pragma clang fp contract(off)
   bool in[V], like[V];
   float d0[V], d1[V], d2[V], pp[V], ri[V];
@@ -117,9 +117,13 @@ __device__ __forceinline__ void forceGroupM(const float4& me, float dfi, int ei,
                             static_cast<uint32_t>(m)) * 16u);
     }
   }
+  float fp[V];
 #pragma unroll
-  for (int u = 0; u < V; ++u)
-    if (in[u]) acc.add(d0[u], d1[u], d2[u], pairFpM(pp[u], ri[u], zc[u], sj[u], si[u], like[u], dfi, nb[u].w, tb.rdr));
+  for (int u = 0; u < V; ++u) {
+    fp[u] = 0.0f;  // the segments of a neighbour beyond the cutoff are not read
+    if (in[u]) fp[u] = pairFpM(pp[u], ri[u], zc[u], sj[u], si[u], like[u], dfi, nb[u].w, tb.rdr);
+  }
+  addGroup3<ACC, V>(acc, ks, in, d0, d1, d2, fp);
 }
 
 // forceGroupM with the Fe-Fe r*phi cubics (zT) and the Fe density slope
@@ -132,7 +136,7 @@ template <int ACC, int V>
 __device__ __forceinline__ void forceGroupL(const float4& me, float dfi, int ei, uint32_t pairBase,
                                             const float4 (&nb)[V], const float4 (&d)[V], float rc2, Rsrc rtab,
                                             Rsrc rtabD, const Tables<float>& tb, const float4* zT,
-                                            const float4* sT, int mlo, Sum3<ACC>& acc) {
+                                            const float4* sT, int mlo, const int* ks, Sum3<ACC>& acc) {
 #pragma clang fp contract(off)
   bool in[V], like[V], fz[V];
   float d0[V], d1[V], d2[V], pp[V], ri[V];
@@ -170,9 +174,13 @@ __device__ __forceinline__ void forceGroupL(const float4& me, float dfi, int ei,
                             static_cast<uint32_t>(mm[u])) * 16u);
     }
   }
+  float fp[V];
 #pragma unroll
-  for (int u = 0; u < V; ++u)
-    if (in[u]) acc.add(d0[u], d1[u], d2[u], pairFpM(pp[u], ri[u], zc[u], sj[u], si[u], like[u], dfi, nb[u].w, tb.rdr));
+  for (int u = 0; u < V; ++u) {
+    fp[u] = 0.0f;  // the segments of a neighbour beyond the cutoff are not read
+    if (in[u]) fp[u] = pairFpM(pp[u], ri[u], zc[u], sj[u], si[u], like[u], dfi, nb[u].w, tb.rdr);
+  }
+  addGroup3<ACC, V>(acc, ks, in, d0, d1, d2, fp);
 }
 
 // After the lattice sum of site s: the inter atoms and, near a far lattice

@@ -271,12 +271,20 @@ struct Sum3<0> {
 template <>
 struct Sum3<1> {
   Sum1<1> a, b, c;
-  double f0 = 0, f1 = 0, f2 = 0;
+  float g0 = 0, g1 = 0, g2 = 0;  // the open chunk of lattice terms (addGroup3)
+  int chunk = 0;                 // its offsets: 8 chunk .. 8 chunk + 7
   __device__ __forceinline__ void add(double d0, double d1, double d2, float fp) {
 #pragma clang fp contract(off)
     a.add(static_cast<float>(d0) * fp);
     b.add(static_cast<float>(d1) * fp);
     c.add(static_cast<float>(d2) * fp);
+  }
+  // a chunk's sum is exactly zero only as the lane's own terms make it
+  __device__ __forceinline__ void flush() {
+    if (g0 != 0.0f) a.add(g0);
+    if (g1 != 0.0f) b.add(g1);
+    if (g2 != 0.0f) c.add(g2);
+    g0 = g1 = g2 = 0.0f;
   }
 };
 template <>
@@ -288,6 +296,44 @@ struct Sum3<2> {
     f2 = fmaf(d2, fp, f2);
   }
 };
+// The force terms of a group of lattice neighbours (in[u]: neighbour u
+// counts; ks[u]: its offset, ascending over a site's groups).  With
+// compensated sums (ACC 1) the terms of the offsets 8t .. 8t+7 are summed in
+// FP32 (fused multiply-adds) and that chunk's sum joins the compensated sum
+// as one term, when the offsets move past the chunk and once more at the end
+// (flushChunk3); a lane adds no empty chunk.  The chunks hang on the offsets
+// alone, not on how they are grouped for the gathers nor on which other
+// sites share the wavefront, so every launch shape gives the same bits.
+// Otherwise term by term.
+template <int ACC, int V>
+__device__ __forceinline__ void addGroup3(Sum3<ACC>& acc, const int* ks, const bool (&in)[V], const float (&d0)[V],
+                                          const float (&d1)[V], const float (&d2)[V], const float (&fp)[V]) {
+#pragma clang fp contract(off)
+  if constexpr (ACC == 1) {
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      const int t = ks[u] >> 3;  // the same in every lane
+      if (t != acc.chunk) {
+        acc.flush();
+        acc.chunk = t;
+      }
+      if (in[u]) {
+        acc.g0 = fmaf(d0[u], fp[u], acc.g0);
+        acc.g1 = fmaf(d1[u], fp[u], acc.g1);
+        acc.g2 = fmaf(d2[u], fp[u], acc.g2);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < V; ++u)
+      if (in[u]) acc.add(d0[u], d1[u], d2[u], fp[u]);
+  }
+}
+template <int ACC>
+__device__ __forceinline__ void flushChunk3(Sum3<ACC>& acc) {
+  if constexpr (ACC == 1) acc.flush();
+}
+
 template <int ACC>
 __device__ __forceinline__ void sum3Value(const Sum3<ACC>& s, double& f0, double& f1, double& f2);
 template <>
