@@ -50,7 +50,8 @@ typedef struct misa_batch misa_batch;
 /* precision of the resident-mode force path */
 #define MISA_PREC_FP64 0     /* FP64 everywhere */
 #define MISA_PREC_MIXED 1    /* FP32 splines and pair terms, FP64 positions/Verlet/sums */
-#define MISA_PREC_MIXED_KAHAN 17   /* MIXED with per-atom sums in FP32 plus Kahan compensation */
+#define MISA_PREC_MIXED_KAHAN 17   /* MIXED with per-atom sums in FP32 plus Kahan compensation (the lattice
+                                      force terms joining it in chunks of eight offsets) */
 #define MISA_PREC_MIXED_F32SUM 33  /* MIXED with plain FP32 per-atom sums */
 
 /* Geometry of the single-rank domain: the whole periodic box.  Mirrors the
