@@ -42,6 +42,8 @@ void oracle_run(void* h, int64_t steps) {
 
 void oracle_set_dt(void* h, double dt) { static_cast<Sim*>(h)->setDt(dt); }
 
+void oracle_plant(void* h, int which) { static_cast<Sim*>(h)->plant = which; }
+
 void oracle_setv(void* h, const int* lat4, const double* dir, double e) { static_cast<Sim*>(h)->setv(lat4, dir, e); }
 
 void oracle_collision(void* h, const int* lat4, const double* dir, double e) {

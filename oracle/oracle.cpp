@@ -513,7 +513,7 @@ void Sim::secondStep() {
 int Sim::decide() {
   inter_ghost_list.clear();  // clearGhost, inter_atom_list.cpp:71-74
   int nflag = 0;
-  const double thr = std::pow(0.2 * d.a, 2.0);
+  const double thr = plant == 1 ? 0.2 * d.a : std::pow(0.2 * d.a, 2.0);
   for (int64_t k = 0; k < d.dbx_sub[2]; k++)
     for (int64_t j = 0; j < d.dbx_sub[1]; j++)
       for (int64_t i = 0; i < d.dbx_sub[0]; i++) {
@@ -525,17 +525,23 @@ int Sim::decide() {
         double dist = (at.x[0] - xt) * (at.x[0] - xt);
         dist += (at.x[1] - yt) * (at.x[1] - yt);
         dist += (at.x[2] - zt) * (at.x[2] - zt);
-        if (dist > thr) {
+        if (plant == 2 ? dist >= thr : dist > thr) {
           inter_list.push_back(at);
           at.type = kInvalid;
           at.v[0] = at.v[1] = at.v[2] = 0;
           nflag = 1;
         }
       }
+  if (plant == 4) inter_list.reverse();
   for (auto it = inter_list.begin(); it != inter_list.end();) {
-    const int64_t n = nearLatIndexInSubBox(it->x);
+    int64_t n = nearLatIndexInSubBox(it->x);
+    if (plant == 5) {
+      int64_t c[3];
+      nearLatCoord(it->x, c);
+      n = lat.index(c[0], c[1], c[2]);
+    }
     AtomElement* near = n == ws::kIndexNotExists ? nullptr : &atoms[n];
-    if (near != nullptr && near->isInter() && isOutBox(near->x) == 0u) {
+    if (near != nullptr && near->isInter() && (plant == 3 || isOutBox(near->x) == 0u)) {
       near->id = it->id;
       near->type = it->type;
       for (int c = 0; c < 3; c++) {
@@ -547,6 +553,7 @@ int Sim::decide() {
       ++it;
     }
   }
+  if (plant == 4) inter_list.reverse();
   return nflag;
 }
 

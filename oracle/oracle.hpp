@@ -171,6 +171,11 @@ struct Sim {
   void firstStep();
   void secondStep();
   int decide();
+  // Planted mistakes in decide, for the tests that show its checks catch
+  // them (0 = the reference's rule): 1 threshold 0.2a left unsquared, 2 >=
+  // in place of >, 3 refill without the stale-position box test, 4 the last
+  // claimant of a vacancy wins, 5 refills into ghost sites.
+  int plant = 0;
   void exchangeAtomFirst();
   void exchangeAtom();
   void exchangeInter();

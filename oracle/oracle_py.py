@@ -40,6 +40,7 @@ def load():
             "oracle_prepare": (None, [vp]),
             "oracle_run": (None, [vp, i64]),
             "oracle_set_dt": (None, [vp, d]),
+            "oracle_plant": (None, [vp, i32]),
             "oracle_setv": (None, [vp, vp, vp, d]),
             "oracle_collision": (None, [vp, vp, vp, d]),
             "oracle_phase": (i32, [vp, i32]),
@@ -185,6 +186,10 @@ class Sim:
 
     def set_dt(self, dt):
         self.L.oracle_set_dt(self.h, dt)
+
+    def plant(self, which):
+        """Plant mistake `which` in decide (0 = none); see Sim::plant."""
+        self.L.oracle_plant(self.h, which)
 
     def phase(self, name):
         return self.L.oracle_phase(self.h, PHASES[name])
