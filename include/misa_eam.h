@@ -161,6 +161,25 @@ int misa_set_neighbour_skin(misa_handle* h, double skin);
 int misa_sim_time(misa_handle* h, double* t_ps, double* dt_last);
 /* Inter-atom list capacity (default 65536 entries). */
 int misa_set_inter_capacity(misa_handle* h, int64_t cap);
+/* Device error flags.  Every call that runs kernels checks them when it
+ * ends: a NaN or infinite force or electron density (MISA_E_DEVICE), an inter
+ * atom outside the ghost-extended box (MISA_E_DEVICE), a full inter-atom list
+ * (MISA_E_CAPACITY).  An electron density beyond the last knot of the F(rho)
+ * table (F' then comes from the table's last segment, as the interpolation
+ * clamps it) is recorded always and fails the call only with the range check
+ * on (on = 1; default off).  The flags stay set until the next upload.  No
+ * counterpart in the reference, whose hooks have no error channel
+ * (arch_imp.h:17-31); its development build aborts on |sum f_x| > 1e-5
+ * (md_simulation.cpp:150-164) and on a wrong F' halo length
+ * (df_embed_packer.cpp:50-53). */
+#define MISA_FLAG_INTER_CAPACITY 1
+#define MISA_FLAG_GHOST_CAPACITY 2
+#define MISA_FLAG_NON_FINITE 4
+#define MISA_FLAG_LEFT_BOX 8
+#define MISA_FLAG_RHO_RANGE 16
+int misa_set_range_check(misa_handle* h, int32_t on);
+/* The flags word as the last checking call read it (MISA_FLAG_* bits). */
+int misa_device_flags(const misa_handle* h, int64_t* flags);
 
 /* clearForce + computeEam on the resident state (simulation.cpp:141-143):
  * the first force evaluation before the step loop. */

@@ -55,6 +55,8 @@ struct misa_handle {
   std::vector<int> event_cls;
   double timings[4] = {0, 0, 0, 0};
   int64_t launches[3] = {0, 0, 0};
+  bool range_check = false;  // a density beyond the F(rho) table fails the call
+  int64_t flags_seen = 0;    // the device flags at the last check
 };
 
 namespace {
@@ -175,11 +177,15 @@ int syncAndCheck(misa_handle* h) {
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   Ctl c;
   HIPCHECK(h, hipMemcpy(&c, h->s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
-  if (c.err_flags & 1) return fail(h, MISA_E_CAPACITY, "inter-atom list capacity exceeded");
-  if (c.err_flags & 2) return fail(h, MISA_E_CAPACITY, "ghost inter-atom capacity exceeded");
-  if (c.err_flags & 4) return fail(h, MISA_E_DEVICE, "non-finite force");
-  if (c.err_flags & 8) return fail(h, MISA_E_DEVICE, "an inter atom left the ghost-extended box");
-  if (c.err_flags & ~int64_t(15)) return fail(h, MISA_E_DEVICE, "device error flag set");
+  h->flags_seen = c.err_flags;
+  if (c.err_flags & kErrInterCap) return fail(h, MISA_E_CAPACITY, "inter-atom list capacity exceeded");
+  if (c.err_flags & kErrGhostCap) return fail(h, MISA_E_CAPACITY, "ghost inter-atom capacity exceeded");
+  if (c.err_flags & kErrNonFinite) return fail(h, MISA_E_DEVICE, "non-finite force or electron density");
+  if (c.err_flags & kErrLeftBox) return fail(h, MISA_E_DEVICE, "an inter atom left the ghost-extended box");
+  if ((c.err_flags & kErrRhoRange) && h->range_check)
+    return fail(h, MISA_E_DEVICE, "electron density beyond the embedding-function table");
+  if (c.err_flags & ~int64_t(kErrInterCap | kErrGhostCap | kErrNonFinite | kErrLeftBox | kErrRhoRange))
+    return fail(h, MISA_E_DEVICE, "device error flag set");
   return MISA_OK;
 }
 
@@ -628,6 +634,7 @@ int misa_upload_state(misa_handle* h, const void* atoms, const void* inter, int6
   launchHaloAll(h->s, h->s.cur, h->stream);
   HIPCHECK(h, hipGetLastError());
   HIPCHECK(h, hipMemsetAsync(h->s.ctl, 0, sizeof(Ctl), h->stream));
+  h->flags_seen = 0;
   h->t_fixed = 0.0;
   HIPCHECK(h, hipMemsetAsync(h->s.in.head, 0xff, static_cast<size_t>(2 * h->s.G.plane) * 4, h->stream));
   if (n_inter > 0) {
@@ -771,6 +778,18 @@ int misa_sim_time(misa_handle* h, double* t_ps, double* dt_last) {
   HIPCHECK(h, hipMemcpy(&c, h->s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
   *t_ps = h->t_fixed + c.t_sim;
   if (dt_last) *dt_last = h->max_disp > 0.0 ? c.dt : h->dt;
+  return MISA_OK;
+}
+
+int misa_set_range_check(misa_handle* h, int32_t on) {
+  if (!h || (on != 0 && on != 1)) return MISA_E_ARG;
+  h->range_check = on != 0;
+  return MISA_OK;
+}
+
+int misa_device_flags(const misa_handle* h, int64_t* flags) {
+  if (!h || !flags) return MISA_E_ARG;
+  *flags = h->flags_seen;
   return MISA_OK;
 }
 

@@ -66,7 +66,7 @@ struct Ctl {
   int64_t n_ghost;      // ghost images
   int64_t n_stage;      // new inter atoms staged by the Verlet kernels
   int64_t n_prev;       // entries recorded in prev_slots
-  int64_t err_flags;    // bit 0: inter capacity, bit 1: ghost capacity, bit 2: non-finite, bit 3: left the box
+  int64_t err_flags;    // kErr* bits below
   int64_t n_far;        // lattice atoms farther than 0.2a from their site, found for the next index
   int64_t n_far_prev;   // the same, as flagged by the last decide
   double dt;            // adaptive step: length of the current step (ps)
@@ -84,6 +84,30 @@ struct Ctl {
   int64_t pad[2];
 };
 static_assert(sizeof(Ctl) == 15 * 8, "Ctl layout");
+
+// Bits of Ctl::err_flags, set by the kernels and read after every call.
+constexpr int64_t kErrInterCap = 1;   // inter-atom list capacity exceeded
+constexpr int64_t kErrGhostCap = 2;   // ghost inter-atom capacity exceeded
+constexpr int64_t kErrNonFinite = 4;  // a force (or density) came out NaN or infinite
+constexpr int64_t kErrLeftBox = 8;    // an inter atom left the ghost-extended box
+constexpr int64_t kErrRhoRange = 16;  // an electron density beyond the last knot of F(rho)
+
+__device__ __forceinline__ void raiseFlag(const Ctl* ctl, int64_t bits) {
+  atomicOr(reinterpret_cast<unsigned long long*>(&const_cast<Ctl*>(ctl)->err_flags),
+           static_cast<unsigned long long>(bits));
+}
+// Non-finite sums raise kErrNonFinite (the sum of the three is NaN or
+// infinite when any of them is; a finite overflow of the sum is as bad).
+__device__ __forceinline__ void checkFinite(const Ctl* ctl, double a, double b, double c) {
+  if (!__builtin_isfinite(a + b + c)) raiseFlag(ctl, kErrNonFinite);
+}
+// The density of a site against the embedding table: NaN or infinite raises
+// kErrNonFinite, beyond the last knot kErrRhoRange (F' then comes from the
+// last segment, as the interpolation clamps it).
+__device__ __forceinline__ void checkRho(const Ctl* ctl, double rho, double rhomax) {
+  if (!__builtin_isfinite(rho)) raiseFlag(ctl, kErrNonFinite);
+  else if (rho > rhomax) raiseFlag(ctl, kErrRhoRange);
+}
 
 // Masses and the force-to-velocity unit factor (atom_types.h:17-19,
 // pre_define.h:11-21) for the kernels that derive the half-kick factors from

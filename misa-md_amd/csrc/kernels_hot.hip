@@ -226,6 +226,7 @@ __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T>
       }
     }
     rho = acc.value();
+    checkRho(iv.ctl, rho, static_cast<double>(tb.rhomax));
     dfi = dEmbed(tb, species(ti), static_cast<T>(rho));
   }
   writeMasks(!isVacant(ti), wv, w0, w1, wmask, !skin, s0, s1, smask);
@@ -437,6 +438,7 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
   }
   double f0, f1, f2;
   sum3Value<ACC>(acc, f0, f1, f2);
+  checkFinite(ctl, f0, f1, f2);
   verletTail<MODE>(G, s, p, cx, cy, cz, ti, ei, xi0, xi1, xi2, f0, f1, f2, xn, v, f, id, vp, st, ctl, vb, db);
   }();
   if (MODE == 0 && vp.vmax) waveMaxTo(vb, vp.vmax);

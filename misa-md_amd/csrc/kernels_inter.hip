@@ -517,6 +517,7 @@ __global__ __launch_bounds__(kIT) void kInterRho(Geom G, Tables<T> tb, InterView
     forBucket(iv, n, ne, [&](int e) { ent(e, true); });
     for (int k = 0; k < full.n[par]; k++) forBucket(iv, n + full.off[par][k], ne, [&](int e) { ent(e, false); });
     in.rho[i] = rho;
+    checkRho(iv.ctl, rho, static_cast<double>(tb.rhomax));
     const T dfi = dEmbed(tb, species(in.type[i]), static_cast<T>(rho));
     if constexpr (sizeof(T) == 8)
       in.df64[i] = dfi;
@@ -571,6 +572,7 @@ __global__ __launch_bounds__(kIT) void kInterForce(Geom G, Tables<T> tb, InterVi
       forBucket(iv, n, ne, [&](int e) { ent(e, true); });
       for (int k = 0; k < full.n[par]; k++) forBucket(iv, n + full.off[par][k], ne, [&](int e) { ent(e, false); });
     }
+    checkFinite(iv.ctl, f0, f1, f2);
     in.f[0][i] = f0;
     in.f[1][i] = f1;
     in.f[2][i] = f2;

@@ -220,6 +220,7 @@ __device__ __forceinline__ void rhoTailM(const Geom& G, int64_t s, int p, int cx
       }
     }
     rho = acc.value();
+    checkRho(iv.ctl, rho, static_cast<double>(tb.rhomax));
     dfi = dEmbed(tb, species(ti), static_cast<float>(rho));
   }
   float* w = reinterpret_cast<float*>(rec) + 3;
@@ -297,6 +298,7 @@ __device__ __forceinline__ void forceTailM(const Geom& G, int64_t s, int p, int 
   }
   double f0, f1, f2;
   sum3Value<ACC>(acc, f0, f1, f2);
+  checkFinite(ctl, f0, f1, f2);
   const StepDt sd = stepDt(vp, ei);
   const double c = sd.c;
   double v0 = v.c[0][s] + c * f0;

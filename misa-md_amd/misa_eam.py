@@ -127,6 +127,8 @@ _SIGS = {
     "misa_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "misa_set_timestep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     "misa_set_inter_capacity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "misa_set_range_check": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "misa_device_flags": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "misa_set_adaptive_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
     "misa_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_void_p)]),
@@ -196,8 +198,14 @@ def load():
     return _lib
 
 
+E_ARG, E_HIP, E_IO, E_DEVICE, E_CAPACITY = -1, -2, -3, -4, -5
+FLAG_INTER_CAPACITY, FLAG_GHOST_CAPACITY, FLAG_NON_FINITE, FLAG_LEFT_BOX, FLAG_RHO_RANGE = 1, 2, 4, 8, 16
+
+
 class MisaError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def _ptr(a):
@@ -228,7 +236,7 @@ class EamDevice:
 
     def _check(self, rc, what):
         if rc != 0:
-            raise MisaError("%s failed (%d): %s" % (what, rc, self.lib.misa_last_error(self.h).decode()))
+            raise MisaError("%s failed (%d): %s" % (what, rc, self.lib.misa_last_error(self.h).decode()), rc)
 
     def close(self):
         if self.h is not None:
@@ -303,6 +311,14 @@ class EamDevice:
 
     def set_inter_capacity(self, cap):
         self._check(self.lib.misa_set_inter_capacity(self.h, int(cap)), "misa_set_inter_capacity")
+
+    def set_range_check(self, on):
+        self._check(self.lib.misa_set_range_check(self.h, int(bool(on))), "misa_set_range_check")
+
+    def device_flags(self):
+        f = ctypes.c_int64()
+        self._check(self.lib.misa_device_flags(self.h, ctypes.byref(f)), "misa_device_flags")
+        return f.value
 
     def set_adaptive_dt(self, dt_max, max_disp):
         self._check(self.lib.misa_set_adaptive_dt(self.h, dt_max, max_disp), "misa_set_adaptive_dt")

@@ -1042,13 +1042,28 @@ void Sim::collisionStep(const int l[4], const double dir[3], double energy) {  /
   computeEam();
 }
 
+// Compensated (Neumaier) sum: the totals below add millions of terms, and a
+// plain running sum would carry a rounding error of order N eps |E|, far
+// above the per-atom bar the device's reduction is held to.
+struct CompSum {
+  double s = 0.0, c = 0.0;
+  CompSum& operator+=(double x) {
+    const double t = s + x;
+    if (std::fabs(s) >= std::fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+    return *this;
+  }
+  double value() const { return s + c; }
+};
+
 // Potential energy E = sum_i F_i(rho_i) + 1/2 sum_{i != j} phi_ij(r_ij) and
 // pair virial W = sum_pairs r_ij . f_ij over the state left by computeEam().
 // Each lattice pair is visited once through the half lists, lattice-inter
 // pairs once from the inter side, inter-inter pairs from both sides at 1/2.
 void Sim::energy(double out[4]) {
   const double rc2 = rc * rc;
-  double e_embed = 0, e_pair = 0, w = 0;
+  CompSum e_embed, e_pair, w;
   const int64_t xs = d.dbx_ghost[0], ys = d.dbx_ghost[1], zs = d.dbx_ghost[2];
   for (int64_t k = zs; k < d.dbx_sub[2] + zs; k++)
     for (int64_t j = ys; j < d.dbx_sub[1] + ys; j++)
@@ -1095,10 +1110,10 @@ void Sim::energy(double out[4]) {
       }
     }
   }
-  out[0] = e_embed;
-  out[1] = e_pair;
-  out[2] = e_embed + e_pair;
-  out[3] = w;
+  out[0] = e_embed.value();
+  out[1] = e_pair.value();
+  out[2] = out[0] + out[1];
+  out[3] = w.value();
 }
 
 }  // namespace oracle
