@@ -51,9 +51,6 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "kforce_mixed_fecuni_200c_traffic.
 
 # FP32 splines/pair terms, FP32 + Kahan per-atom sums
 MIXED = M.PREC_MIXED_KAHAN
-# the mixed force pass on a box of this size: persistent blocks with the
-# Fe-Fe spline cubics in LDS (the library's default for large boxes)
-MIXED_FORCE_KERNEL = "kForceL"
 MVV2E = 1.0364269e-4  # amu A^2/ps^2 -> eV (pre_define.h)
 
 
@@ -78,6 +75,56 @@ def equilibrate(dev, steps):
         dev.step(50)
         dev.rescale(300.0)
     return dev.download()
+
+
+def develop_cascade(dev, state, pka_site, pka_energy, t_target):
+    """Untimed: from the equilibrated state, the recoil and FP32-mixed
+    adaptive steps until `t_target` ps of cascade; returns the state (with its
+    inter atoms), the simulated time and the steps taken."""
+    dev.upload(*state)
+    dev.set_precision(MIXED)
+    dev.set_adaptive_dt(0.001, 0.0)
+    dev.compute_forces()
+    dev.collision_step(pka_site, PKA_DIR, pka_energy)
+    dev.set_adaptive_dt(0.001, 0.01)
+    steps = 0
+    while True:
+        dev.step(250)
+        steps += 250
+        t, _ = dev.sim_time()
+        if steps % 1000 == 0:
+            th = dev.thermo()
+            log("cascade: %d steps, t %.4f ps, %d interstitials" % (steps, t, th["n_inter"]))
+        if t >= t_target:
+            break
+    return dev.download(), t, steps
+
+
+def run_developed(dev, state, mode, steps, warmup):
+    """From a developed-cascade state (inter atoms included): warm-up steps,
+    then `steps` timed adaptive steps, HIP events around every launch."""
+    dev.upload(*state)
+    dev.set_precision(mode)
+    dev.set_adaptive_dt(0.001, 0.01)
+    dev.compute_forces()
+    if warmup:
+        dev.step(warmup)
+    natoms = int(np.prod(dev.cells)) * 2
+    th0 = dev.thermo()
+    e_start = total_energy(th0, natoms)
+    t0_sim, _ = dev.sim_time()
+    dev.set_profiling(True)
+    t0 = time.perf_counter()
+    dev.step(steps)  # returns after the device has finished
+    secs = time.perf_counter() - t0
+    tim = dev.last_timings()
+    dev.set_profiling(False)
+    t_sim, dt_last = dev.sim_time()
+    th = dev.thermo()
+    return secs, tim, {"t_window_ps": t_sim - t0_sim, "dt_last_ps": dt_last, "n_inter_start": th0["n_inter"],
+                       "n_inter": th["n_inter"], "n_vacancy": th["n_vacancy"], "max_speed": th["max_speed"],
+                       "e_total_timed_start_eV_per_atom": e_start,
+                       "e_total_timed_end_eV_per_atom": total_energy(th, natoms)}
 
 
 def run_precision(dev, state, mode, steps, warmup, pka_energy, pka_site, pkas=None):
@@ -272,26 +319,61 @@ def large_workload(pot, args):
     }
 
 
-def cpu_baseline(setfl):
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count(), len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(setfl, setfl_fe, cells):
+    """The oracle (the reference's single-rank step restated: half lists,
+    104-B AoS, reference phase order) on one host thread: the timed workload
+    (FeCuNi 985:10:5 at `cells`^3, 300 K) for one step after its first force
+    evaluation, beside the 20^3 Fe box (600 K, 100 steps) and the 100^3
+    FeCuNi box (3 steps).  `value` is the workload's rate."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
     exe = oracle_py.BASELINE_BIN
     if not os.path.exists(exe):
         return None
-    cells, steps = 50, 20
-    out = subprocess.run([exe, setfl, str(cells), str(cells), str(cells), "985", "10", "5", "300", "2020",
-                          str(steps)], check=True, capture_output=True, text=True, timeout=600)
-    r = json.loads(out.stdout.strip().splitlines()[-1])
-    return {"value": r["atom_steps_per_s"], "unit": "atom-steps/s", "cores": 1, "kind": "port",
-            "sample": "oracle (reference single-rank step restated: half lists, 104-B AoS, reference phase order) "
-                      "on FeCuNi 985:10:5 BCC %d^3 cells (%d atoms), 300 K, %d steps, one thread; %s" % (
-                          cells, 2 * cells ** 3, steps, "phase seconds " + json.dumps(r["phase_s"]))}
+
+    def run(pot, n, ratio, t, seed, steps):
+        out = subprocess.run([exe, pot, str(n), str(n), str(n)] + [str(r) for r in ratio] +
+                             [str(t), str(seed), str(steps)], check=True, capture_output=True, text=True,
+                             timeout=900)
+        return json.loads(out.stdout.strip().splitlines()[-1])
+
+    samples = {}
+    for name, pot, n, ratio, t, seed, steps in (
+            ("fe_20c", setfl_fe, 20, (1, 0, 0), 600.0, 466953, 100),
+            ("fecuni_100c", setfl, 100, (985, 10, 5), 300.0, 2020, 3),
+            ("fecuni_%dc" % cells, setfl, cells, (985, 10, 5), 300.0, 2020, 1)):
+        log("cpu baseline: %s, %d steps" % (name, steps))
+        r = run(pot, n, ratio, t, seed, steps)
+        samples[name] = {"atoms": int(r["atoms"]), "steps": steps, "seconds": r["seconds"],
+                         "atom_steps_per_s": r["atom_steps_per_s"], "phase_s": r["phase_s"]}
+    model, ncpu, avail = host_cpu()
+    main = samples["fecuni_%dc" % cells]
+    return {"value": main["atom_steps_per_s"], "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": "oracle on one host thread (g++ -O3, x86-64-v3): BCC FeCuNi 985:10:5 %d^3 cells (%d atoms), "
+                      "300 K, 1 equilibrium step timed after the first force evaluation; beside it 20^3 Fe 600 K "
+                      "100 steps and 100^3 FeCuNi 3 steps" % (cells, main["atoms"]),
+            "host_cpu": model, "host_nproc": ncpu, "host_cpus_available": avail, "samples": samples}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--early-steps", type=int, default=20, help="timed steps right after the recoil (side key)")
+    ap.add_argument("--develop-ps", type=float, default=0.3, help="untimed cascade time before the timed steps")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cells", type=int, default=200)
     ap.add_argument("--equil", type=int, default=100, help="untimed equilibration steps")
@@ -321,6 +403,8 @@ def main():
     with tempfile.TemporaryDirectory() as tmp:
         pot = os.path.join(tmp, "fecuni.eam.alloy")
         setfl_gen.write_setfl(pot, ("Fe", "Cu", "Ni"))
+        pot_fe = os.path.join(tmp, "fe.eam.alloy")
+        setfl_gen.write_setfl(pot_fe, ("Fe",))
         log("building %d^3 cells (%d atoms)" % (n, natoms))
         dev = M.EamDevice((n, n, n))
         dev.make_offsets()
@@ -330,11 +414,21 @@ def main():
         log("equilibration")
         state = equilibrate(dev, args.equil)
         site = tuple(n // 2 for _ in range(3)) + (0,)
-        log("fp64 run")
-        s64, t64, c64 = run_precision(dev, state, M.PREC_FP64, args.steps, args.warmup, args.pka_energy, site)
-        log("mixed run")
-        s32, t32, c32 = run_precision(dev, state, MIXED, args.steps, args.warmup, args.pka_energy, site)
+        early = {}
+        for name, mode in (("fp64", M.PREC_FP64), ("mixed", MIXED)):
+            log("%s run, first steps after the recoil" % name)
+            secs, tim, cas = run_precision(dev, state, mode, args.early_steps, args.warmup, args.pka_energy, site)
+            early[name] = {"value": natoms * args.early_steps / secs, "ms_per_step": secs * 1e3 / args.early_steps,
+                           "steps": args.early_steps, "cascade": cas}
+        log("developing the cascade to %.2f ps (FP32-mixed, untimed)" % args.develop_ps)
+        dstate, t_dev, dev_steps = develop_cascade(dev, state, site, args.pka_energy, args.develop_ps)
         del state
+        runs = {}
+        for name, mode in (("fp64", M.PREC_FP64), ("mixed", MIXED)):
+            log("%s run, developed cascade" % name)
+            runs[name] = run_developed(dev, dstate, mode, args.steps, args.warmup)
+        del dstate
+        launched = dev.force_pass()
         log("force error")
         fmax, frms, f_scale = force_error(dev)
         th = dev.thermo()
@@ -343,8 +437,10 @@ def main():
         if args.drift_steps > 0:
             log("energy drift")
             drift = energy_drift(pot, args.drift_cells, args.drift_steps)
-        cpu = None if args.no_cpu else cpu_baseline(pot)
+        cpu = None if args.no_cpu else cpu_baseline(pot, pot_fe, n)
 
+    s32, t32, c32 = runs["mixed"]
+    s64, t64, c64 = runs["fp64"]
     rate32 = natoms * args.steps / s32
     rate64 = natoms * args.steps / s64
     avg_force_ms = t32["force_ms"] / max(1, t32["launches"][1])
@@ -356,14 +452,19 @@ def main():
     def gbs(nbytes, ms):
         return natoms * nbytes / (ms * 1e-3) / 1e9
 
+    def shares(t, secs):
+        total = secs * 1e3
+        return {"rho_pass": t["rho_df_ms"] / total, "force_pass": t["force_ms"] / total,
+                "inter_and_ownership": t["other_ms"] / total}
+
     kernels = {
-        "mixed": {"force_kernel": MIXED_FORCE_KERNEL, "force_ms": avg_force_ms, "force_GBs": achieved,
+        "mixed": {"force_kernel": launched, "force_ms": avg_force_ms, "force_GBs": achieved,
                   "rho_kernel": "kRhoDfM", "rho_ms": avg_rho_ms, "rho_GBs": gbs(RHO_BYTES["mixed"], avg_rho_ms),
-                  "other_ms_per_step": t32["other_ms"] / args.steps},
+                  "other_ms_per_step": t32["other_ms"] / args.steps, "share_of_step": shares(t32, s32)},
         "fp64": {"force_kernel": "kForce", "force_ms": avg_force64,
                  "force_GBs": gbs(FORCE_BYTES["fp64"], avg_force64),
                  "rho_kernel": "kRhoDf", "rho_ms": avg_rho64, "rho_GBs": gbs(RHO_BYTES["fp64"], avg_rho64),
-                 "other_ms_per_step": t64["other_ms"] / args.steps},
+                 "other_ms_per_step": t64["other_ms"] / args.steps, "share_of_step": shares(t64, s64)},
     }
     res = {
         "metric": METRIC,
@@ -380,27 +481,32 @@ def main():
         "data": "synthetic",
         "config": {"workload": ("collision cascade: BCC FeCuNi %d^3 cells (%d atoms), Fe:Cu:Ni 985:10:5, "
                                 "equilibrated at 300 K (%d steps of 1 fs), PKA %.0f eV at %s along [1 3 5], "
-                                "adaptive dt = min(1 fs, 0.01 A / max|v|), timed steps %d after %d warm-up; "
-                                "synthetic 3-element setfl Nr=Nrho=5000 rc 5.6 A")
-                               % (n, natoms, args.equil, args.pka_energy, str(site), args.steps, args.warmup)
-                               if args.pka_energy > 0 else
-                               "BCC FeCuNi %d^3 cells (%d atoms), Fe:Cu:Ni 985:10:5, 300 K, dt 1 fs, synthetic "
-                               "3-element setfl Nr=Nrho=5000 rc 5.6 A" % (n, natoms),
+                                "adaptive dt = min(1 fs, 0.01 A / max|v|), developed for %.3f ps (%d steps, "
+                                "untimed), then %d timed steps after %d warm-up; synthetic 3-element setfl "
+                                "Nr=Nrho=5000 rc 5.6 A")
+                               % (n, natoms, args.equil, args.pka_energy, str(site), t_dev, dev_steps, args.steps,
+                                  args.warmup),
                    "atoms": natoms, "parallelism": "single GPU"},
-        "cascade": {"mixed": c32, "fp64": c64},
+        "cascade": {"developed_at_ps": t_dev, "mixed": c32, "fp64": c64},
         "fp64": {"value": rate64, "unit": "atom-steps/s", "ms_per_step": s64 * 1e3 / args.steps},
+        "early_window": early,
         "kernels": kernels,
         "mixed_over_fp64": rate32 / rate64,
-        "force_err_mixed_vs_fp64": {"max": fmax, "rms": frms, "rms_force_eV_per_A": f_scale},
+        "force_err_mixed_vs_fp64": {"max": fmax, "rms": frms, "rms_force_eV_per_A": f_scale,
+                                    "within_1e-5": frms <= 1e-5},
         "energy_drift": drift,
-        "roofline": {"bound": "hbm", "kernel": MIXED_FORCE_KERNEL + " (force + Verlet pass, mixed)",
+        "roofline": {"bound": "hbm", "kernel": launched + " (force + Verlet pass, mixed)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_bytes(), "algorithmic_bytes_per_launch": natoms * FORCE_BYTES["mixed"],
+                     "traffic": traffic_bytes(), "traffic_source": "committed rocprofv3 counter summary " +
+                     os.path.relpath(TRAFFIC_FILE, ROOT) + " (FETCH_SIZE + WRITE_SIZE, calibrated on a copy)",
+                     "algorithmic_bytes_per_launch": natoms * FORCE_BYTES["mixed"],
                      "algorithmic_bytes_per_atom": FORCE_BYTES["mixed"], "avg_launch_ms": avg_force_ms},
         "step_roofline_frac": rate32 * STEP_BYTES["mixed"] / (HBM_PEAK_GBS * 1e9),
         "thermo": {"temperature": th["temperature"], "e_potential_per_atom": th["e_potential"] / natoms},
         "cpu_baseline": cpu,
     }
+    if frms > 1e-5:
+        res["failed"] = "mixed-vs-FP64 RMS force error %.3e exceeds 1e-5" % frms
     print(json.dumps(res))
 
 

@@ -157,6 +157,12 @@ int misa_set_adaptive_dt(misa_handle* h, double dt_max, double max_disp);
  * density pass visits only those.  The pairs and their order do not change,
  * so neither do the results. */
 int misa_set_neighbour_skin(misa_handle* h, double skin);
+/* Ownership events since the last upload (atom::decide, atom.cpp:21-84):
+ * [0] lattice atoms taken off their sites (moved more than 0.2a), [1] inter
+ * atoms put back on a vacant site of their Wigner-Seitz cell, [2] of those
+ * the ones farther than 0.2a from it (their neighbourhoods then visit the
+ * whole full neighbour list for one step). */
+int misa_defect_counters(misa_handle* h, int64_t out[3]);
 /* Simulated time (ps) since the last upload, and the length of the last step. */
 int misa_sim_time(misa_handle* h, double* t_ps, double* dt_last);
 /* Inter-atom list capacity (default 65536 entries). */
@@ -242,6 +248,15 @@ int misa_get_launch_shape(const misa_handle* h, int32_t* ty, int32_t* gathers_rh
 #define MISA_LDS_ON 1
 #define MISA_LDS_OFF 2
 int misa_set_lds_tables(misa_handle* h, int32_t mode);
+/* The force pass the last force evaluation launched (-1 before any):
+ * kForce (FP64), kForceM (mixed, spline segments from global memory) or
+ * kForceL (mixed, persistent blocks with the majority species' segments in
+ * LDS; the library falls back to kForceM when the device refuses its LDS
+ * size). */
+#define MISA_FORCE_PASS_FP64 0
+#define MISA_FORCE_PASS_MIXED 1
+#define MISA_FORCE_PASS_MIXED_LDS 2
+int misa_force_pass_used(const misa_handle* h, int32_t* kind);
 
 /* ---- host mirror of libcomm's domain at one rank ----
  *

@@ -55,6 +55,7 @@ struct misa_handle {
   std::vector<int> event_cls;
   double timings[4] = {0, 0, 0, 0};
   int64_t launches[3] = {0, 0, 0};
+  int force_pass = -1;       // MISA_FORCE_PASS_* of the last force pass launched
   bool range_check = false;  // a density beyond the F(rho) table fails the call
   int64_t flags_seen = 0;    // the device flags at the last check
 };
@@ -818,7 +819,7 @@ static void forcePass(misa_handle* h, int lattice_mode, int inter_mode, bool sto
   }
   {
     Timer t(h, 1);
-    launchForce(h->s, h->lat, h->t64, h->t32, mixed, h->acc, lattice_mode, vp, h->stream);
+    h->force_pass = launchForce(h->s, h->lat, h->t64, h->t32, mixed, h->acc, lattice_mode, vp, h->stream);
   }
 }
 
@@ -1330,6 +1331,23 @@ int misa_set_lds_tables(misa_handle* h, int32_t mode) {
     return fail(h, MISA_E_ARG, "LDS table mode must be MISA_LDS_AUTO, MISA_LDS_ON or MISA_LDS_OFF");
   h->s.lds = mode;
   h->gen++;
+  return MISA_OK;
+}
+
+int misa_defect_counters(misa_handle* h, int64_t out[3]) {
+  if (!h || !out) return MISA_E_ARG;
+  HIPCHECK(h, hipSetDevice(h->device));
+  Ctl c;
+  HIPCHECK(h, hipMemcpy(&c, h->s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+  out[0] = c.staged_total;
+  out[1] = c.refill_total;
+  out[2] = c.far_total;
+  return MISA_OK;
+}
+
+int misa_force_pass_used(const misa_handle* h, int32_t* kind) {
+  if (!h || !kind) return MISA_E_ARG;
+  *kind = h->force_pass;
   return MISA_OK;
 }
 

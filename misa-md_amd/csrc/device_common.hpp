@@ -81,9 +81,13 @@ struct Ctl {
   unsigned long long dispmax2;
   int32_t skin_mode;
   int32_t rescan;
-  int64_t pad[2];
+  // since the upload: lattice atoms taken off their sites, inter atoms put
+  // back on a vacant site, and of those the ones farther than 0.2a from it
+  int64_t staged_total;
+  int64_t refill_total;
+  int64_t far_total;
 };
-static_assert(sizeof(Ctl) == 15 * 8, "Ctl layout");
+static_assert(sizeof(Ctl) == 16 * 8, "Ctl layout");
 
 // Bits of Ctl::err_flags, set by the kernels and read after every call.
 constexpr int64_t kErrInterCap = 1;   // inter-atom list capacity exceeded

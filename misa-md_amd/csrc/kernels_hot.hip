@@ -736,8 +736,9 @@ void launchRhoDf(const DevState& s, const LatOffsets& o, const Tables<double>& t
   }
 }
 
-void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t64, const Tables<float>& t32,
-                 bool mixed, int acc, int mode, const VerletParams& vp, hipStream_t st) {
+int launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t64, const Tables<float>& t32,
+                bool mixed, int acc, int mode, const VerletParams& vp, hipStream_t st) {
+  int kind = mixed ? 1 : 0;
   const unsigned grid = siteGrid(s.G);
   const CXPtr x = cxp(s.x[s.cur]);
   const XPtr xn = xp(s.x[1 - s.cur]);
@@ -753,6 +754,7 @@ void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t
     static const bool ldsReady =                                                                              \
         setLdsBytes(reinterpret_cast<const void*>(&kForceL<A, M, U>), kForceLdsBytes);                       \
     if (lds && ldsReady) {                                                                                    \
+      kind = 2;                                                                                               \
       kForceL<A, M, U><<<persistentGrid(s), kBlockP, lk.bytes, st>>>(                                         \
           s.G, o, t32, x, xn, s.type, s.rec[s.cur], s.rec[1 - s.cur], s.dl, s.wmask, xp(s.v), xp(s.f), s.id, vp, \
           sa, s.ctl, iv, lk.mlo, lk.nkn);                                                                     \
@@ -788,6 +790,7 @@ void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t
 #undef MISA_FORCE
 #undef MISA_FORCE_M
 #undef MISA_FORCE_MU
+  return kind;
 }
 
 }  // namespace misa

@@ -165,33 +165,138 @@ __device__ __forceinline__ void forSlotAndImages(const Geom& G, int64_t slot, Fn
   forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) { fn(q); });
 }
 
+// Ascending block sort of n 64-bit keys at a (LDS or global memory): a
+// bitonic network over the next power of two, padded with ~0.
+__device__ void blockSortU64(uint64_t* a, int n) {
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int i = n + static_cast<int>(threadIdx.x); i < P; i += kDT) a[i] = ~0ull;
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < P; i += kDT) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const uint64_t x = a[i], y = a[ixj];
+          const bool up = (i & k) == 0;
+          if ((x > y) == up) {
+            a[i] = y;
+            a[ixj] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+}
+
+// Keys sorted in LDS when they fit (kSortLds of them), else in place in
+// global memory.
+constexpr int kSortLds = 8192;
+__device__ void sortKeys(uint64_t* g, int n, uint64_t* lds) {
+  int P = 1;
+  while (P < n) P <<= 1;
+  if (P > kSortLds) {
+    blockSortU64(g, n);
+    return;
+  }
+  for (int i = threadIdx.x; i < n; i += kDT) lds[i] = g[i];
+  __syncthreads();
+  blockSortU64(lds, n);
+  for (int i = threadIdx.x; i < n; i += kDT) g[i] = lds[i];
+  __syncthreads();
+}
+
+// The 8-bit type array in 32-bit words: set or clear bits of one byte with a
+// word atomic, so that many threads may flag neighbouring sites at once.
+__device__ __forceinline__ void typeBitsOr(int8_t* t, int64_t i, int8_t bits) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(t + i);
+  const unsigned sh = static_cast<unsigned>(a & 3u) * 8u;
+  atomicOr(reinterpret_cast<unsigned*>(a & ~uintptr_t(3)), static_cast<unsigned>(static_cast<uint8_t>(bits)) << sh);
+}
+__device__ __forceinline__ void typeBitsAnd(int8_t* t, int64_t i, int8_t keep) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(t + i);
+  const unsigned sh = static_cast<unsigned>(a & 3u) * 8u;
+  const unsigned m = ~(static_cast<unsigned>(static_cast<uint8_t>(~keep)) << sh);
+  atomicAnd(reinterpret_cast<unsigned*>(a & ~uintptr_t(3)), m);
+}
+
+// Calls fn(t) for neighbour k (-1: the slot itself) of `slot` in the full
+// list when that neighbour is a sub-box site.
+template <typename Fn>
+__device__ __forceinline__ void fullNeighbour(const Geom& G, const FullPlaneOffsets& full, int64_t slot, int k,
+                                              Fn fn) {
+  const int par = slot >= G.plane ? 1 : 0;
+  if (k >= full.n[par]) return;
+  const int64_t t = k < 0 ? slot : slot + full.off[par][k];
+  if (t >= 0 && t < 2 * G.plane && isSubBoxSlot(G, t)) fn(t);
+}
+
+// Grid of the flag kernels: grid-stride over (entry, neighbour) items whose
+// count is known only on the device.
+constexpr int kFlagBlocks = 1024;
+
+// Decide, step 1 (before the list work): the previous index's bucket heads
+// and the flags it set, kNearInter around its entries and kWide around its
+// far lattice atoms and their images (atom.cpp:21-84 rebuilds the index each
+// step; inter_atom_list.cpp:27-45).
+__global__ __launch_bounds__(kBlock) void kUnflag(Geom G, InterDev in, int8_t* ltype, const Ctl* ctl) {
+  const int nprev = static_cast<int>(ctl->n_prev);
+  const int nfarPrev = static_cast<int>(ctl->n_far_prev);
+  const FullPlaneOffsets& full = *in.full;
+  const int nfull = 1 + max(full.n[0], full.n[1]);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  for (int64_t q = t0; q < nprev; q += stride) in.head[in.prev_slot[q]] = -1;
+  for (int64_t q = t0; q < static_cast<int64_t>(nprev) * nfull; q += stride) {
+    const int e = static_cast<int>(q / nfull), k = static_cast<int>(q % nfull) - 1;
+    fullNeighbour(G, full, in.prev_slot[e], k, [&](int64_t t) {
+      if (ltype[t] & kNearInter) typeBitsAnd(ltype, t, static_cast<int8_t>(~kNearInter));
+    });
+  }
+  for (int64_t q = t0; q < static_cast<int64_t>(nfarPrev) * nfull; q += stride) {
+    const int e = static_cast<int>(q / nfull), k = static_cast<int>(q % nfull) - 1;
+    forSlotAndImages(G, in.far_prev[e], [&](int64_t slot) {
+      fullNeighbour(G, full, slot, k, [&](int64_t t) {
+        const int8_t v = ltype[t];
+        if (!isVacant(v) && (v & kWide)) typeBitsAnd(ltype, t, static_cast<int8_t>(~kWide));
+      });
+    });
+  }
+}
+
+// Decide, step 2: the list work in one workgroup (the lists hold the
+// interstitials and the atoms that moved more than 0.2a this step: hundreds
+// to thousands in a developed cascade).
 __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
   const Geom& G = A.G;
   InterDev in = A.in;
   const int tid = threadIdx.x;
   int nl = static_cast<int>(A.ctl->n_inter);
-  const int nprev = static_cast<int>(A.ctl->n_prev);
-  const int nfarPrev = static_cast<int>(A.ctl->n_far_prev);
   __shared__ int sFar;  // far lattice atoms for this index (kFarScan's in mode 1)
+  __shared__ uint64_t sKeys[kSortLds];
   if (tid == 0) sFar = static_cast<int>(min(A.ctl->n_far, in.cap));
   __syncthreads();
   unsigned long long* err = reinterpret_cast<unsigned long long*>(&A.ctl->err_flags);
-  int refills = 0;
+  int refills = 0, staged = 0;
 
   if (A.mode == 0) {
     // P1: append the new inter atoms in the order of the sites they left
-    // (atom.cpp:28-53 walks the sub-box z -> y -> x and appends)
+    // (atom.cpp:28-53 walks the sub-box z -> y -> x and appends): sort the
+    // staged atoms by the reference linear index of their site
     int ns = static_cast<int>(A.ctl->n_stage);
     if (ns > in.cap) ns = static_cast<int>(in.cap);
     if (nl + ns > in.cap) {
       if (tid == 0) setFlag(err, 1ull);
       ns = static_cast<int>(in.cap) - nl;
     }
-    for (int i = tid; i < ns; i += kDT) {
-      const int64_t key = in.stage_key[i];
-      int rank = 0;
-      for (int j = 0; j < ns; j++) rank += in.stage_key[j] < key ? 1 : 0;
-      const int dst = nl + rank;
+    staged = ns;
+    for (int i = tid; i < ns; i += kDT)
+      in.keys[i] = (static_cast<uint64_t>(in.stage_key[i]) << 21) | static_cast<uint64_t>(i);
+    __syncthreads();
+    sortKeys(in.keys, ns, sKeys);
+    for (int r = tid; r < ns; r += kDT) {
+      const int i = static_cast<int>(in.keys[r] & ((1u << 21) - 1));
+      const int dst = nl + r;
       in.id[dst] = in.stage_id[i];
       in.type[dst] = static_cast<int8_t>(species(in.stage_type[i]));
       for (int k = 0; k < 3; k++) {
@@ -330,113 +435,26 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
     }
   }
 
-  // P6: reset the previous index and the flags around its entries
-  for (int q = tid; q < nprev; q += kDT) {
-    const int64_t slot = in.prev_slot[q];
-    in.head[slot] = -1;
-  }
-  __syncthreads();
-  const int nfull = 1 + max(in.full->n[0], in.full->n[1]);
-  for (int q = tid; q < nprev * nfull; q += kDT) {
-    const int e = q / nfull;
-    const int k = q % nfull - 1;
-    const int64_t slot = in.prev_slot[e];
-    const int par = slot >= G.plane ? 1 : 0;
-    if (k >= in.full->n[par]) continue;
-    const int64_t t = k < 0 ? slot : slot + in.full->off[par][k];
-    if (t >= 0 && t < 2 * G.plane && isSubBoxSlot(G, t)) {
-      const int8_t v = A.ltype[t];
-      if (v & kNearInter) A.ltype[t] = static_cast<int8_t>(v & ~kNearInter);
-    }
-  }
-  __syncthreads();
-  // the same for the wide-list flags around the previous far lattice atoms
-  // and their periodic images
-  for (int q = tid; q < nfarPrev * nfull; q += kDT) {
-    const int e = q / nfull;
-    const int k = q % nfull - 1;
-    forSlotAndImages(G, in.far_prev[e], [&](int64_t slot) {
-      const int par = slot >= G.plane ? 1 : 0;
-      if (k >= in.full->n[par]) return;
-      const int64_t t = k < 0 ? slot : slot + in.full->off[par][k];
-      if (t >= 0 && t < 2 * G.plane && isSubBoxSlot(G, t)) {
-        const int8_t v = A.ltype[t];
-        if (!isVacant(v) && (v & kWide)) A.ltype[t] = static_cast<int8_t>(v & ~kWide);
-      }
-    });
-  }
-  __syncthreads();
+  // P6: the index: keys = (slot << 24 | entry) sorted, so buckets come in slot
+  // order and each bucket in insertion order (inter_atom_list.cpp:27-45)
   const int nfar = min(sFar, static_cast<int>(in.cap));
-  // keys = (slot << 24 | entry), sorted: buckets in slot order, insertion order within
-  int P = 1;
-  while (P < ne) P <<= 1;
-  for (int e = tid; e < P; e += kDT) {
-    uint64_t key = ~0ull;
-    if (e < ne) {
-      int64_t X, y, z;
-      int64_t slot = nearExtSlot(G, in.e_x[0][e], in.e_x[1][e], in.e_x[2][e], X, y, z);
-      if (slot < 0) {
-        setFlag(err, 8ull);
-        slot = 0;
-      }
-      in.e_slot[e] = slot;
-      key = (static_cast<uint64_t>(slot) << kEntryBits) | static_cast<uint64_t>(e);
+  for (int e = tid; e < ne; e += kDT) {
+    int64_t X, y, z;
+    int64_t slot = nearExtSlot(G, in.e_x[0][e], in.e_x[1][e], in.e_x[2][e], X, y, z);
+    if (slot < 0) {
+      setFlag(err, 8ull);
+      slot = 0;
     }
-    in.keys[e] = key;
+    in.e_slot[e] = slot;
+    in.keys[e] = (static_cast<uint64_t>(slot) << kEntryBits) | static_cast<uint64_t>(e);
   }
   __syncthreads();
-  for (int k = 2; k <= P; k <<= 1)
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = tid; i < P; i += kDT) {
-        const int ixj = i ^ j;
-        if (ixj > i) {
-          const uint64_t a = in.keys[i], b = in.keys[ixj];
-          const bool up = (i & k) == 0;
-          if ((a > b) == up) {
-            in.keys[i] = b;
-            in.keys[ixj] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
+  sortKeys(in.keys, ne, sKeys);
   for (int q = tid; q < ne; q += kDT) {
     const int64_t slot = static_cast<int64_t>(in.keys[q] >> kEntryBits);
     if (q == 0 || static_cast<int64_t>(in.keys[q - 1] >> kEntryBits) != slot) in.head[slot] = q;
     in.prev_slot[q] = slot;
   }
-  __syncthreads();
-  // P7: flag the sub-box sites within the full neighbour list of every entry
-  for (int q = tid; q < ne * nfull; q += kDT) {
-    const int e = q / nfull;
-    const int k = q % nfull - 1;
-    const int64_t slot = in.e_slot[e];
-    const int par = slot >= G.plane ? 1 : 0;
-    if (k >= in.full->n[par]) continue;
-    const int64_t t = k < 0 ? slot : slot + in.full->off[par][k];
-    if (t >= 0 && t < 2 * G.plane && isSubBoxSlot(G, t)) {
-      const int8_t v = A.ltype[t];
-      if (!(v & kNearInter)) A.ltype[t] = static_cast<int8_t>(v | kNearInter);
-    }
-  }
-  __syncthreads();
-  // P8: the sites whose sums must visit the whole full list: the full
-  // neighbourhood of every lattice atom farther than 0.2a from its site, and
-  // of each of its periodic images
-  for (int q = tid; q < nfar * nfull; q += kDT) {
-    const int e = q / nfull;
-    const int k = q % nfull - 1;
-    forSlotAndImages(G, in.far_new[e], [&](int64_t slot) {
-      const int par = slot >= G.plane ? 1 : 0;
-      if (k >= in.full->n[par]) return;
-      const int64_t t = k < 0 ? slot : slot + in.full->off[par][k];
-      if (t >= 0 && t < 2 * G.plane && isSubBoxSlot(G, t)) {
-        const int8_t v = A.ltype[t];
-        if (!isVacant(v) && !(v & kWide)) A.ltype[t] = static_cast<int8_t>(v | kWide);
-      }
-    });
-  }
-  for (int q = tid; q < nfar; q += kDT) in.far_prev[q] = in.far_new[q];
   __syncthreads();
   if (tid == 0) {
     A.ctl->n_inter = nl;
@@ -445,6 +463,9 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
     A.ctl->n_prev = ne;
     A.ctl->n_far_prev = nfar;
     A.ctl->n_far = 0;
+    A.ctl->staged_total += staged;
+    A.ctl->refill_total += refills;
+    A.ctl->far_total += A.mode == 0 ? nfar : 0;
     // Neighbour skin of the density pass (both precisions).  Since the last full scan no
     // lattice atom has moved farther than disp_acc, so no pair has come closer
     // by more than 2 disp_acc: an offset beyond cutoff + skin for every lane
@@ -460,6 +481,36 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
     c->dispmax2 = 0ull;
     c->rescan = 0;
   }
+}
+
+// Decide, step 3: flag the sub-box sites within the full neighbour list of
+// every entry (kNearInter: their sums visit the inter atoms), and those
+// within the full neighbourhood of every lattice atom farther than 0.2a from
+// its site and of its periodic images (kWide: their sums visit the whole full
+// list); the far list becomes the one the next decide clears.
+__global__ __launch_bounds__(kBlock) void kFlag(Geom G, InterDev in, int8_t* ltype, const Ctl* ctl) {
+  const int ne = static_cast<int>(ctl->n_prev);
+  const int nfar = static_cast<int>(ctl->n_far_prev);
+  const FullPlaneOffsets& full = *in.full;
+  const int nfull = 1 + max(full.n[0], full.n[1]);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  for (int64_t q = t0; q < static_cast<int64_t>(ne) * nfull; q += stride) {
+    const int e = static_cast<int>(q / nfull), k = static_cast<int>(q % nfull) - 1;
+    fullNeighbour(G, full, in.e_slot[e], k, [&](int64_t t) {
+      if (!(ltype[t] & kNearInter)) typeBitsOr(ltype, t, kNearInter);
+    });
+  }
+  for (int64_t q = t0; q < static_cast<int64_t>(nfar) * nfull; q += stride) {
+    const int e = static_cast<int>(q / nfull), k = static_cast<int>(q % nfull) - 1;
+    forSlotAndImages(G, in.far_new[e], [&](int64_t slot) {
+      fullNeighbour(G, full, slot, k, [&](int64_t t) {
+        const int8_t v = ltype[t];
+        if (!isVacant(v) && !(v & kWide)) typeBitsOr(ltype, t, kWide);
+      });
+    });
+  }
+  for (int64_t q = t0; q < nfar; q += stride) in.far_prev[q] = in.far_new[q];
 }
 
 __global__ void kRescan(Ctl* ctl) { ctl->rescan = 1; }
@@ -482,47 +533,60 @@ __global__ __launch_bounds__(kBlock) void kFarScan(Geom G, CXPtr x, const int8_t
 // per inter atom: rho/F', force + half kick, first half step
 // ---------------------------------------------------------------------------
 
+// Per inter atom one wavefront: lane l takes the sites of full-list items
+// l - 1, l + 63, l + 127, ... (item -1 is the nearest site itself) and the
+// inter atoms in their buckets; the lanes' partial sums then meet in a fixed
+// shuffle tree, so the result does not depend on the launch.
+constexpr int kWaveBlocks = 256;  // blocks of 4 wavefronts, grid-stride over the inter atoms
+
+__device__ __forceinline__ double waveSum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // atom::interRho (atom.cpp:194-284): lattice (nearest site, full list), same
-// bucket without itself, then the buckets of the full list.
+// bucket without itself, then the buckets of the full list; F' at once.
 template <typename T>
 __global__ __launch_bounds__(kIT) void kInterRho(Geom G, Tables<T> tb, InterView iv, CXPtr lx,
                                                  const int8_t* __restrict__ ltype, InterDev in) {
   const int nl = static_cast<int>(iv.ctl->n_inter);
   const int ne = nl + static_cast<int>(iv.ctl->n_ghost);
   const FullPlaneOffsets& full = *iv.full;
-  for (int i = blockIdx.x * kIT + threadIdx.x; i < nl; i += kIT * kIB) {
+  const int lane = threadIdx.x & 63;
+  const int w0 = blockIdx.x * (kIT / 64) + static_cast<int>(threadIdx.x >> 6);
+  for (int i = w0; i < nl; i += kWaveBlocks * (kIT / 64)) {
     const int64_t n = in.e_slot[i];
     if (!isSubBoxSlot(G, n)) continue;
     const double xi0 = in.x[0][i], xi1 = in.x[1][i], xi2 = in.x[2][i];
     const uint64_t myid = in.id[i];
-    double rho = 0.0;
     const int par = n >= G.plane ? 1 : 0;
-    auto lat = [&](int64_t j) {
+    double rho = 0.0;
+    for (int k = lane - 1; k < full.n[par]; k += 64) {
+      const int64_t j = k < 0 ? n : n + full.off[par][k];
       const int8_t tj = ltype[j];
-      if (isVacant(tj)) return;
-      const double d0 = xi0 - lx.c[0][j], d1 = xi1 - lx.c[1][j], d2 = xi2 - lx.c[2][j];
-      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-      if (r2 < G.rc2) rho += rhoPair(tb, species(tj), r2);
-    };
-    auto ent = [&](int e, bool skip_self) {
-      const int src = iv.e_src[e];
-      const uint64_t eid = e < nl ? iv.id[src] : 0;  // ghost images carry id 0
-      if (skip_self && eid == myid) return;
-      const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
-      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-      if (r2 < G.rc2) rho += rhoPair(tb, species(iv.type[src]), r2);
-    };
-    lat(n);
-    for (int k = 0; k < full.n[par]; k++) lat(n + full.off[par][k]);
-    forBucket(iv, n, ne, [&](int e) { ent(e, true); });
-    for (int k = 0; k < full.n[par]; k++) forBucket(iv, n + full.off[par][k], ne, [&](int e) { ent(e, false); });
-    in.rho[i] = rho;
-    checkRho(iv.ctl, rho, static_cast<double>(tb.rhomax));
-    const T dfi = dEmbed(tb, species(in.type[i]), static_cast<T>(rho));
-    if constexpr (sizeof(T) == 8)
-      in.df64[i] = dfi;
-    else
-      in.df32[i] = dfi;
+      if (!isVacant(tj)) {
+        const double d0 = xi0 - lx.c[0][j], d1 = xi1 - lx.c[1][j], d2 = xi2 - lx.c[2][j];
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        if (r2 < G.rc2) rho += rhoPair(tb, species(tj), r2);
+      }
+      forBucket(iv, j, ne, [&](int e) {
+        const int src = iv.e_src[e];
+        if (k < 0 && e < nl && iv.id[src] == myid) return;  // itself (ghost images carry no id)
+        const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        if (r2 < G.rc2) rho += rhoPair(tb, species(iv.type[src]), r2);
+      });
+    }
+    rho = waveSum(rho);
+    if (lane == 0) {
+      in.rho[i] = rho;
+      checkRho(iv.ctl, rho, static_cast<double>(tb.rhomax));
+      const T dfi = dEmbed(tb, species(in.type[i]), static_cast<T>(rho));
+      if constexpr (sizeof(T) == 8)
+        in.df64[i] = dfi;
+      else
+        in.df32[i] = dfi;
+    }
   }
 }
 
@@ -534,7 +598,9 @@ __global__ __launch_bounds__(kIT) void kInterForce(Geom G, Tables<T> tb, InterVi
   const int nl = static_cast<int>(iv.ctl->n_inter);
   const int ne = nl + static_cast<int>(iv.ctl->n_ghost);
   const FullPlaneOffsets& full = *iv.full;
-  for (int i = blockIdx.x * kIT + threadIdx.x; i < nl; i += kIT * kIB) {
+  const int lane = threadIdx.x & 63;
+  const int w0 = blockIdx.x * (kIT / 64) + static_cast<int>(threadIdx.x >> 6);
+  for (int i = w0; i < nl; i += kWaveBlocks * (kIT / 64)) {
     const int64_t n = in.e_slot[i];
     double f0 = 0, f1 = 0, f2 = 0;
     const int ei = species(in.type[i]);
@@ -543,45 +609,49 @@ __global__ __launch_bounds__(kIT) void kInterForce(Geom G, Tables<T> tb, InterVi
       const uint64_t myid = in.id[i];
       const T dfi = dfOf<T>(iv, i);
       const int par = n >= G.plane ? 1 : 0;
-      auto lat = [&](int64_t j) {
+      for (int k = lane - 1; k < full.n[par]; k += 64) {
+        const int64_t j = k < 0 ? n : n + full.off[par][k];
         const int8_t tj = ltype[j];
-        const double d0 = xi0 - lx.c[0][j], d1 = xi1 - lx.c[1][j], d2 = xi2 - lx.c[2][j];
-        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        if (r2 < G.rc2 && !isVacant(tj)) {
-          const double fp = static_cast<double>(pairForce(tb, ei, species(tj), r2, dfi, ldf[j * ldfStride]));
-          f0 += d0 * fp;
-          f1 += d1 * fp;
-          f2 += d2 * fp;
+        {
+          const double d0 = xi0 - lx.c[0][j], d1 = xi1 - lx.c[1][j], d2 = xi2 - lx.c[2][j];
+          const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+          if (r2 < G.rc2 && !isVacant(tj)) {
+            const double fp = static_cast<double>(pairForce(tb, ei, species(tj), r2, dfi, ldf[j * ldfStride]));
+            f0 += d0 * fp;
+            f1 += d1 * fp;
+            f2 += d2 * fp;
+          }
         }
-      };
-      auto ent = [&](int e, bool skip_self) {
-        const int src = iv.e_src[e];
-        const uint64_t eid = e < nl ? iv.id[src] : 0;
-        if (skip_self && eid == myid) return;
-        const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
-        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        if (r2 < G.rc2) {
-          const double fp = static_cast<double>(pairForce(tb, ei, species(iv.type[src]), r2, dfi, dfOf<T>(iv, src)));
-          f0 += d0 * fp;
-          f1 += d1 * fp;
-          f2 += d2 * fp;
-        }
-      };
-      lat(n);
-      for (int k = 0; k < full.n[par]; k++) lat(n + full.off[par][k]);
-      forBucket(iv, n, ne, [&](int e) { ent(e, true); });
-      for (int k = 0; k < full.n[par]; k++) forBucket(iv, n + full.off[par][k], ne, [&](int e) { ent(e, false); });
+        forBucket(iv, j, ne, [&](int e) {
+          const int src = iv.e_src[e];
+          if (k < 0 && e < nl && iv.id[src] == myid) return;
+          const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
+          const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+          if (r2 < G.rc2) {
+            const double fp =
+                static_cast<double>(pairForce(tb, ei, species(iv.type[src]), r2, dfi, dfOf<T>(iv, src)));
+            f0 += d0 * fp;
+            f1 += d1 * fp;
+            f2 += d2 * fp;
+          }
+        });
+      }
+      f0 = waveSum(f0);
+      f1 = waveSum(f1);
+      f2 = waveSum(f2);
     }
-    checkFinite(iv.ctl, f0, f1, f2);
-    in.f[0][i] = f0;
-    in.f[1][i] = f1;
-    in.f[2][i] = f2;
-    if (MODE == 0) {  // secondstep, each product and sum rounded separately
+    if (lane == 0) {
+      checkFinite(iv.ctl, f0, f1, f2);
+      in.f[0][i] = f0;
+      in.f[1][i] = f1;
+      in.f[2][i] = f2;
+      if (MODE == 0) {  // secondstep, each product and sum rounded separately
 #pragma clang fp contract(off)
-      const double c = stepDt(vp, ei).c;
-      in.v[0][i] += c * f0;
-      in.v[1][i] += c * f1;
-      in.v[2][i] += c * f2;
+        const double c = stepDt(vp, ei).c;
+        in.v[0][i] += c * f0;
+        in.v[1][i] += c * f1;
+        in.v[2][i] += c * f2;
+      }
     }
   }
 }
@@ -849,7 +919,9 @@ void launchDecide(const DevState& s, int mode, bool mixed, hipStream_t st) {
   A.in = s.in;
   A.ctl = s.ctl;
   A.skin = s.skin;
+  kUnflag<<<kFlagBlocks, kBlock, 0, st>>>(s.G, s.in, s.type, s.ctl);
   kDecide<<<1, kDT, 0, st>>>(A);
+  kFlag<<<kFlagBlocks, kBlock, 0, st>>>(s.G, s.in, s.type, s.ctl);
 }
 
 void launchRescan(const DevState& s, hipStream_t st) { kRescan<<<1, 1, 0, st>>>(s.ctl); }
@@ -858,9 +930,9 @@ void launchInterRho(const DevState& s, const Tables<double>& t64, const Tables<f
                     hipStream_t st) {
   const InterView iv = interView(s, mixed);
   if (mixed)
-    kInterRho<float><<<kIB, kIT, 0, st>>>(s.G, t32, iv, cxp(s.x[s.cur]), s.type, s.in);
+    kInterRho<float><<<kWaveBlocks, kIT, 0, st>>>(s.G, t32, iv, cxp(s.x[s.cur]), s.type, s.in);
   else
-    kInterRho<double><<<kIB, kIT, 0, st>>>(s.G, t64, iv, cxp(s.x[s.cur]), s.type, s.in);
+    kInterRho<double><<<kWaveBlocks, kIT, 0, st>>>(s.G, t64, iv, cxp(s.x[s.cur]), s.type, s.in);
 }
 
 void launchInterForce(const DevState& s, const Tables<double>& t64, const Tables<float>& t32, bool mixed, int mode,
@@ -870,14 +942,14 @@ void launchInterForce(const DevState& s, const Tables<double>& t64, const Tables
   if (mixed) {  // lattice F' is the w component of the site records
     const float* ldf = reinterpret_cast<const float*>(s.rec[s.cur]) + 3;
     if (mode == 0)
-      kInterForce<float, 0><<<kIB, kIT, 0, st>>>(s.G, t32, iv, lx, s.type, ldf, 4, s.in, vp);
+      kInterForce<float, 0><<<kWaveBlocks, kIT, 0, st>>>(s.G, t32, iv, lx, s.type, ldf, 4, s.in, vp);
     else
-      kInterForce<float, 2><<<kIB, kIT, 0, st>>>(s.G, t32, iv, lx, s.type, ldf, 4, s.in, vp);
+      kInterForce<float, 2><<<kWaveBlocks, kIT, 0, st>>>(s.G, t32, iv, lx, s.type, ldf, 4, s.in, vp);
   } else {
     if (mode == 0)
-      kInterForce<double, 0><<<kIB, kIT, 0, st>>>(s.G, t64, iv, lx, s.type, s.df64, 1, s.in, vp);
+      kInterForce<double, 0><<<kWaveBlocks, kIT, 0, st>>>(s.G, t64, iv, lx, s.type, s.df64, 1, s.in, vp);
     else
-      kInterForce<double, 2><<<kIB, kIT, 0, st>>>(s.G, t64, iv, lx, s.type, s.df64, 1, s.in, vp);
+      kInterForce<double, 2><<<kWaveBlocks, kIT, 0, st>>>(s.G, t64, iv, lx, s.type, s.df64, 1, s.in, vp);
   }
 }
 

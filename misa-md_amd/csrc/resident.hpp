@@ -111,8 +111,10 @@ void launchDownload(const DevState& s, AtomElement* aos_dev, int64_t sx, int64_t
 void launchHaloAll(const DevState& s, int buf, hipStream_t st);
 void launchRhoDf(const DevState& s, const LatOffsets& o, const Tables<double>& t64, const Tables<float>& t32,
                  bool mixed, int acc, bool store_rho, hipStream_t st);
-void launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t64, const Tables<float>& t32,
-                 bool mixed, int acc, int mode, const VerletParams& vp, hipStream_t st);
+// Returns the force pass launched: 0 kForce (FP64), 1 kForceM, 2 kForceL (the
+// values of MISA_FORCE_PASS_*).
+int launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t64, const Tables<float>& t32,
+                bool mixed, int acc, int mode, const VerletParams& vp, hipStream_t st);
 void launchFirst(const DevState& s, bool mixed, const VerletParams& vp, hipStream_t st);
 // Site records of buffer `buf` from x and type over every slot (mixed mode).
 void launchPackRec(const DevState& s, int buf, hipStream_t st);

@@ -139,6 +139,8 @@ _SIGS = {
                                      ctypes.POINTER(ctypes.c_double)]),
     "misa_set_launch_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "misa_set_lds_tables": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "misa_force_pass_used": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "misa_defect_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "misa_get_launch_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
                                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "misa_sim_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
@@ -339,6 +341,18 @@ class EamDevice:
     def set_lds_tables(self, mode):
         """LDS_AUTO, LDS_ON or LDS_OFF: the mixed force pass with its majority-species tables in LDS."""
         self._check(self.lib.misa_set_lds_tables(self.h, int(mode)), "misa_set_lds_tables")
+
+    def defect_counters(self):
+        """{staged, refilled, refilled_far} since the last upload."""
+        out = (ctypes.c_int64 * 3)()
+        self._check(self.lib.misa_defect_counters(self.h, out), "misa_defect_counters")
+        return {"staged": out[0], "refilled": out[1], "refilled_far": out[2]}
+
+    def force_pass(self):
+        """Name of the force kernel the last force evaluation launched."""
+        k = ctypes.c_int32()
+        self._check(self.lib.misa_force_pass_used(self.h, ctypes.byref(k)), "misa_force_pass_used")
+        return {0: "kForce", 1: "kForceM", 2: "kForceL"}.get(k.value, "none")
 
     def launch_shape(self):
         a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
