@@ -122,6 +122,18 @@ int misa_get_potential_table(const misa_handle* h, int kind, int idx, double* ou
 int misa_lat_rho(misa_handle* h, void* atoms, double cutoff_radius);
 int misa_lat_df(misa_handle* h, void* atoms, double cutoff_radius);
 int misa_lat_force(misa_handle* h, void* atoms, double cutoff_radius);
+/* The same three hooks (which = 0 rho, 1 df, 2 force) without any device
+ * allocation.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc), since misa_lat_rho/df/force stage the
+ * array in a device buffer that their first call allocates.  This variant
+ * stages it in a fixed 32 MiB array that the library holds on every GPU it
+ * runs on until the process ends, so boxes of more than 32 MiB / 104 B =
+ * 322,638 ghost-extended slots (about 48^3 cells) are refused with
+ * MISA_E_ARG.  All handles on one GPU share that array: calls that overlap
+ * on one GPU (from several host threads) race.  Each call overwrites the
+ * whole staged array, so nothing carries over between calls. */
+int misa_lat_hook_noalloc(misa_handle* h, int which, void* atoms, double cutoff_radius);
 
 /* ---- resident mode ---- */
 

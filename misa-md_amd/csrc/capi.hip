@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -282,6 +283,32 @@ struct Timer {
   }
 };
 
+// Staging array of misa_lat_hook_noalloc: a fixed device array per GPU in
+// place of the buffer the first hook call allocates.  Host code never names
+// it in a kernel argument or copy: `scratch` on the host is only a stand-in
+// symbol, and the array's device address on each GPU comes from
+// hipGetSymbolAddress, looked up once per device at misa_create.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void* scratchDev[kMaxDevices] = {};
+static std::mutex scratchLock;
+
+// The device address of `scratch` on the current device `device` (looked up
+// on the first call for that device).
+static hipError_t scratchOf(int device, void** ptr) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(scratchLock);
+  if (!scratchDev[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    scratchDev[device] = p;
+  }
+  *ptr = scratchDev[device];
+  return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -294,7 +321,8 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
     return MISA_E_ARG;
   auto* h = new misa_handle();
   h->device = device;
-  if (hipSetDevice(device) != hipSuccess) {
+  void* stage = nullptr;  // the staging array of misa_lat_hook_noalloc on this device
+  if (hipSetDevice(device) != hipSuccess || scratchOf(device, &stage) != hipSuccess) {
     delete h;
     return MISA_E_HIP;
   }
@@ -600,6 +628,27 @@ static int hookCall(misa_handle* h, int which, void* atoms, double rc) {
   launchHook(which, h->box, rc, h->full, h->t64, h->aos_dev, h->stream);
   HIPCHECK(h, hipGetLastError());
   HIPCHECK(h, hipMemcpyAsync(atoms, h->aos_dev, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  return MISA_OK;
+}
+
+int misa_lat_hook_noalloc(misa_handle* h, int which, void* atoms, double rc) {
+  if (!h || !atoms || which < 0 || which > 2) return MISA_E_ARG;
+  if (!h->have_offsets || !h->have_pot) return fail(h, MISA_E_ARG, "offsets and potential must be set first");
+  const size_t bytes = static_cast<size_t>(h->box.slots()) * sizeof(AtomElement);
+  if (bytes > kScratchBytes)
+    return fail(h, MISA_E_ARG, "box of " + std::to_string(h->box.slots()) + " slots (" + std::to_string(bytes) +
+                                   " bytes) exceeds the " + std::to_string(kScratchBytes) +
+                                   "-byte staging array of misa_lat_hook_noalloc");
+  HIPCHECK(h, hipSetDevice(h->device));
+  void* stage = nullptr;
+  HIPCHECK(h, scratchOf(h->device, &stage));
+  AtomElement* a = static_cast<AtomElement*>(stage);
+  // the whole array is overwritten by the copy: nothing in it needs clearing
+  HIPCHECK(h, hipMemcpyAsync(a, atoms, bytes, hipMemcpyHostToDevice, h->stream));
+  launchHook(which, h->box, rc, h->full, h->t64, a, h->stream);
+  HIPCHECK(h, hipGetLastError());
+  HIPCHECK(h, hipMemcpyAsync(atoms, a, bytes, hipMemcpyDeviceToHost, h->stream));
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   return MISA_OK;
 }
