@@ -18,8 +18,8 @@
 // type byte and the atom id; mixed precision adds 16-byte site records
 // (site_common.hpp).  The type byte holds the species in bits 0-1, bit 6
 // marks a site with an inter atom in its 229-site neighbourhood, bit 5 a site
-// near a lattice atom far from its site (kWide), and a negative value marks a
-// vacancy (atom_type::INVALID).
+// near a lattice atom far from its site (kWide), bit 4 that far atom itself
+// (kFar), and a negative value marks a vacancy (atom_type::INVALID).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -37,6 +37,10 @@ constexpr int8_t kNearInter = 0x40;
 // can sit anywhere in that site's Wigner-Seitz cell): its sums also visit
 // the full-list offsets the pruned list leaves out.
 constexpr int8_t kWide = 0x20;
+// The lattice atom farther than 0.2a from its site itself (and its periodic
+// images): the only targets beyond the pruned list that a kWide site's sums
+// can reach, unless the site carries kFar itself.
+constexpr int8_t kFar = 0x10;
 
 // Site traversal of the resident kernels: each XCD takes a z-slab of both
 // parity planes and walks it in y-tiles of `ty` rows (z inner), so the
@@ -162,6 +166,9 @@ constexpr int kPairRec = 8;
 constexpr int kRhoRec = 4;
 
 __device__ __forceinline__ bool isVacant(int8_t t) { return t < 0; }
+// A full-list offset beyond the pruned list can hold a pair within the
+// cutoff only when the site or the occupied target is a far lattice atom.
+__device__ __forceinline__ bool wideTarget(int8_t ti, int8_t tj) { return !(tj < 0) && ((ti | tj) & kFar) != 0; }
 __device__ __forceinline__ int species(int8_t t) { return t & 3; }
 __device__ __forceinline__ int pairIdx(int a, int b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; }
 

@@ -137,7 +137,9 @@ __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T>
   int p, cx, cy, cz;
   if (!siteOfThread(G, s, p, cx, cy, cz)) return;
   const int8_t ti = type[s];
-  const bool skin = __builtin_amdgcn_readfirstlane(iv.ctl->skin_mode) != 0;
+  // a wavefront next to a far lattice atom (kWide) scans in full and records its masks afresh
+  const bool skin =
+      __builtin_amdgcn_readfirstlane(iv.ctl->skin_mode) != 0 && !__ballot(!isVacant(ti) && (ti & kWide));
   const uint32_t wv = waveId();
   T dfi = T(0);
   double rho = 0.0;
@@ -220,9 +222,10 @@ __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T>
       for (int q = 0; q < full.nw[p]; q++) {
         const int64_t j = s + full.woff[p][q];
         const int8_t tj = type[j];
+        if (!wideTarget(ti, tj)) continue;
         const double d0 = xi0 - x.c[0][j], d1 = xi1 - x.c[1][j], d2 = xi2 - x.c[2][j];
         const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        if (!isVacant(tj) && r2 < G.rc2) acc.add(rhoPair(tb, species(tj), r2));
+        if (r2 < G.rc2) acc.add(rhoPair(tb, species(tj), r2));
       }
     }
     rho = acc.value();
@@ -431,9 +434,10 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
     for (int q = 0; q < full.nw[p]; q++) {
       const int64_t j = s + full.woff[p][q];
       const int8_t tj = type[j];
+      if (!wideTarget(ti, tj)) continue;
       const double e0 = xi0 - x.c[0][j], e1 = xi1 - x.c[1][j], e2 = xi2 - x.c[2][j];
       const double r2 = e0 * e0 + e1 * e1 + e2 * e2;
-      if (!isVacant(tj) && r2 < G.rc2) acc.add(e0, e1, e2, pairForce(tb, ei, species(tj), r2, dfi, df[j]));
+      if (r2 < G.rc2) acc.add(e0, e1, e2, pairForce(tb, ei, species(tj), r2, dfi, df[j]));
     }
   }
   double f0, f1, f2;
@@ -466,7 +470,9 @@ __global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<f
   int p, cx, cy, cz;
   if (!siteOfThread(G, s, p, cx, cy, cz)) return;
   const int8_t ti = type[s];
-  const bool skin = __builtin_amdgcn_readfirstlane(iv.ctl->skin_mode) != 0;
+  // a wavefront next to a far lattice atom (kWide) scans in full and records its masks afresh
+  const bool skin =
+      __builtin_amdgcn_readfirstlane(iv.ctl->skin_mode) != 0 && !__ballot(!isVacant(ti) && (ti & kWide));
   const uint32_t wv = waveId();
   Sum1<ACC> acc;
   uint64_t w0 = 0, w1 = 0;  // offsets in range for some lane (uniform among the lanes that gather)
@@ -607,7 +613,7 @@ __global__ __launch_bounds__(kBlockP) void kForceL(Geom G, LatOffsets O, Tables<
                                                    const uint64_t* __restrict__ id, VerletParams vp, StageArgs st,
                                                    Ctl* ctl, InterView iv, int mlo, int nkn) {
   __shared__ float4 offT[2 * kMaxLatOffsets];  // {lattice vector, record shift * 16}
-  extern __shared__ float4 knT[];              // [nkn] Fe-Fe r*phi cubics, then [nkn] Fe slope cubics
+  extern __shared__ float4 knT[];              // per knot: Fe-Fe r*phi cubic, Fe density slope cubic
   for (int i = threadIdx.x; i < 2 * kMaxLatOffsets; i += kBlockP) {
     const int q = i / kMaxLatOffsets, k = i - q * kMaxLatOffsets;
     float4 d = dl[i];
@@ -617,12 +623,10 @@ __global__ __launch_bounds__(kBlockP) void kForceL(Geom G, LatOffsets O, Tables<
   const float4* z4 = reinterpret_cast<const float4*>(tb.pairZ);  // row (Fe, Fe) is the first
   const float4* s4 = reinterpret_cast<const float4*>(tb.rhoD);   // row Fe is the first
   for (int i = threadIdx.x; i < nkn; i += kBlockP) {
-    knT[i] = z4[mlo + i];
-    knT[nkn + i] = s4[mlo + i];
+    knT[2 * i] = z4[mlo + i];
+    knT[2 * i + 1] = s4[mlo + i];
   }
   __syncthreads();
-  const float4* zT = knT;
-  const float4* sT = knT + nkn;
   const Rsrc rr = rsrcBelow(rec, O.lo);
   const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairZ));
   const Rsrc rtabD = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoD));
@@ -654,7 +658,7 @@ __global__ __launch_bounds__(kBlockP) void kForceL(Geom G, LatOffsets O, Tables<
         d[u] = T[ks[u]];
         nb[u] = ldRec4(rr, o16 + __float_as_uint(d[u].w), 0);
       }
-      forceGroupL<ACC, V>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, zT, sT, mlo, ks, acc);
+      forceGroupL<ACC, V>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, knT, mlo, ks, acc);
     };
     forMaskGroups<U>(uniform64(wmask[2 * wv]), uniform64(wmask[2 * wv + 1]), [&](const int* ks, int nv) {
       if (nv == U) group(std::integral_constant<int, U>{}, ks);

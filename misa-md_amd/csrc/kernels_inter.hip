@@ -134,7 +134,7 @@ __device__ __forceinline__ bool inForwardRegion(const Geom& G, int dim, int dir,
 }
 
 // Type (and, for `pos`, position) of a lattice slot to its periodic images;
-// in mixed precision also the site record: NaN for a vacancy, else the
+// in mixed precision also the site record: vacantRec for a vacancy, else the
 // displacement of the (new) atom from its site.
 __device__ __forceinline__ void writeLatticeImages(const DecideArgs& A, int64_t slot, bool pos, int8_t t) {
   int p, cx, cy, cz;
@@ -262,6 +262,11 @@ __global__ __launch_bounds__(kBlock) void kUnflag(Geom G, InterDev in, int8_t* l
       });
     });
   }
+  for (int64_t q = t0; q < nfarPrev; q += stride)
+    forSlotAndImages(G, in.far_prev[q], [&](int64_t slot) {
+      const int8_t v = ltype[slot];
+      if (!isVacant(v) && (v & kFar)) typeBitsAnd(ltype, slot, static_cast<int8_t>(~kFar));
+    });
 }
 
 // Decide, step 2: the list work in one workgroup (the lists hold the
@@ -272,9 +277,13 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
   InterDev in = A.in;
   const int tid = threadIdx.x;
   int nl = static_cast<int>(A.ctl->n_inter);
-  __shared__ int sFar;  // far lattice atoms for this index (kFarScan's in mode 1)
+  __shared__ int sFar;   // far lattice atoms for this index (kFarScan's in mode 1)
+  __shared__ int sNear;  // refilled atoms within 0.2a of their site
   __shared__ uint64_t sKeys[kSortLds];
-  if (tid == 0) sFar = static_cast<int>(min(A.ctl->n_far, in.cap));
+  if (tid == 0) {
+    sFar = static_cast<int>(min(A.ctl->n_far, in.cap));
+    sNear = 0;
+  }
   __syncthreads();
   unsigned long long* err = reinterpret_cast<unsigned long long*>(&A.ctl->err_flags);
   int refills = 0, staged = 0;
@@ -340,6 +349,8 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
           const int q = atomicAdd(&sFar, 1);
           if (q < in.cap) in.far_new[q] = t;
           else setFlag(err, 1ull);
+        } else {
+          atomicAdd(&sNear, 1);
         }
       }
     }
@@ -470,11 +481,13 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
     // lattice atom has moved farther than disp_acc, so no pair has come closer
     // by more than 2 disp_acc: an offset beyond cutoff + skin for every lane
     // then is beyond the cutoff now while disp_acc < skin / 2.  A refill puts
-    // a new atom on a site, and the first force pass (mode 1) follows an
-    // upload: both need a full scan.
+    // a new atom on a site: when it lands farther than 0.2a from the site, its
+    // whole neighbourhood carries kWide, and the wavefronts holding such
+    // sites scan in full and record their masks afresh; any other refill, and
+    // the first force pass after an upload (mode 1), need a full scan of all.
     Ctl* c = A.ctl;
     const double acc = c->disp_acc + sqrt(__longlong_as_double(static_cast<long long>(c->dispmax2)));
-    const bool full = A.mode != 0 || !(A.skin > 0.0) || c->rescan != 0 || refills > 0 ||
+    const bool full = A.mode != 0 || !(A.skin > 0.0) || c->rescan != 0 || sNear > 0 ||
                       acc >= 0.5 * A.skin - kSkinMargin;
     c->skin_mode = full ? 0 : 1;
     c->disp_acc = full ? 0.0 : acc;
@@ -510,7 +523,13 @@ __global__ __launch_bounds__(kBlock) void kFlag(Geom G, InterDev in, int8_t* lty
       });
     });
   }
-  for (int64_t q = t0; q < nfar; q += stride) in.far_prev[q] = in.far_new[q];
+  for (int64_t q = t0; q < nfar; q += stride) {
+    forSlotAndImages(G, in.far_new[q], [&](int64_t slot) {
+      const int8_t v = ltype[slot];
+      if (!isVacant(v) && !(v & kFar)) typeBitsOr(ltype, slot, kFar);
+    });
+    in.far_prev[q] = in.far_new[q];
+  }
 }
 
 __global__ void kRescan(Ctl* ctl) { ctl->rescan = 1; }

@@ -256,6 +256,7 @@ __global__ __launch_bounds__(kBlock) void kEnergy(Geom G, LatOffsets O, const Fu
       for (int k = 0; k < nk; ++k) {
         const int64_t j = s + (k < O.n[p] ? O.off[p][k] : full->woff[p][k - O.n[p]]);
         const int8_t tj = type[j];
+        if (k >= O.n[p] && !wideTarget(ti, tj)) continue;
         const double d0 = xi0 - x.c[0][j], d1 = xi1 - x.c[1][j], d2 = xi2 - x.c[2][j];
         const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
         if (!isVacant(tj) && r2 < G.rc2) {

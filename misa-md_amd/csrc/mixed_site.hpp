@@ -78,8 +78,12 @@ __device__ __forceinline__ void rhoGroupM(const float4& me, const float4 (&nb)[V
 __device__ __forceinline__ float pairFpM(float q, float rinv, const float4& z, const f32x3& sj, const f32x3& si,
                                          bool like, float dfi, float dfj, float rdr) {
 #pragma clang fp contract(off)  // explicit FMAs only: every instantiation rounds alike
-  const float z2 = fmaf(fmaf(fmaf(z.x, q, z.y), q, z.z), q, z.w);
-  const float z2p = fmaf(fmaf(3.0f * z.x, q, 2.0f * z.y), q, z.z) * rdr;
+  // c3 q^3 + c4 q^2 + c5 q + c6 and its slope 3 c3 q^2 + 2 c4 q + c5 from
+  // one chain: a = c3 q + c4, b = a q + c5, slope = (c3 q + a) q + b
+  const float a = fmaf(z.x, q, z.y);
+  const float b = fmaf(a, q, z.z);
+  const float z2 = fmaf(b, q, z.w);
+  const float z2p = fmaf(fmaf(z.x, q, a), q, b) * rdr;
   const float rhojp = fmaf(fmaf(sj.x, q, sj.y), q, sj.z);
   const float rhoip = like ? rhojp : fmaf(fmaf(si.x, q, si.y), q, si.z);
   const float phi = z2 * rinv;
@@ -126,8 +130,9 @@ __device__ __forceinline__ void forceGroupM(const float4& me, float dfi, int ei,
   addGroup3<ACC, V>(acc, ks, in, d0, d1, d2, fp);
 }
 
-// forceGroupM with the Fe-Fe r*phi cubics (zT) and the Fe density slope
-// cubics (sT) from knot mlo on in LDS: every lane reads its segments there,
+// forceGroupM with the Fe-Fe r*phi cubics and the Fe density slope cubics
+// from knot mlo on in LDS (knT, the two of a knot side by side): every lane
+// reads its segments there,
 // at a clamped knot; a lane whose pair involves another species, or lies
 // below knot mlo, then re-reads what it needs from global memory, in a
 // branch that only a wavefront with such a lane enters (after the LDS reads,
@@ -135,8 +140,8 @@ __device__ __forceinline__ void forceGroupM(const float4& me, float dfi, int ei,
 template <int ACC, int V>
 __device__ __forceinline__ void forceGroupL(const float4& me, float dfi, int ei, uint32_t pairBase,
                                             const float4 (&nb)[V], const float4 (&d)[V], float rc2, Rsrc rtab,
-                                            Rsrc rtabD, const Tables<float>& tb, const float4* zT,
-                                            const float4* sT, int mlo, const int* ks, Sum3<ACC>& acc) {
+                                            Rsrc rtabD, const Tables<float>& tb, const float4* knT, int mlo,
+                                            const int* ks, Sum3<ACC>& acc) {
 #pragma clang fp contract(off)
   bool in[V], like[V], fz[V];
   float d0[V], d1[V], d2[V], pp[V], ri[V];
@@ -153,9 +158,9 @@ __device__ __forceinline__ void forceGroupL(const float4& me, float dfi, int ei,
     mm[u] = knotM(r2 * ri[u], tb.rdr, tb.nr, pp[u]);
     ejv[u] = static_cast<uint32_t>(recSpecies(nb[u].x));
     like[u] = ejv[u] == static_cast<uint32_t>(ei);
-    const int j = max(mm[u] - mlo, 0);
-    zc[u] = zT[j];
-    const float4 s4 = sT[j];
+    const float4* k2 = knT + 2 * max(mm[u] - mlo, 0);  // {r*phi cubic, density slope cubic} of the knot
+    zc[u] = k2[0];
+    const float4 s4 = k2[1];
     sj[u] = f32x3{s4.x, s4.y, s4.z};
     si[u] = sj[u];
     fz[u] = in[u] && ((static_cast<uint32_t>(ei) | ejv[u]) != 0u || mm[u] < mlo);
@@ -213,9 +218,10 @@ __device__ __forceinline__ void rhoTailM(const Geom& G, int64_t s, int p, int cx
         for (int q = 0; q < full.nw[p]; q++) {
           const int64_t j = s + full.woff[p][q];
           const int8_t tj = type[j];
+          if (!wideTarget(ti, tj)) continue;
           const double d0 = xi0 - x.c[0][j], d1 = xi1 - x.c[1][j], d2 = xi2 - x.c[2][j];
           const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-          if (!isVacant(tj) && r2 < G.rc2) acc.add(rhoPair(tb, species(tj), r2));
+          if (r2 < G.rc2) acc.add(rhoPair(tb, species(tj), r2));
         }
       }
     }
@@ -233,7 +239,7 @@ __device__ __forceinline__ void rhoTailM(const Geom& G, int64_t s, int p, int cx
 }
 
 // A vacancy in the force pass: f = 0 (clearForce, and latForce skips it), or
-// in the fused pass its stale position and NaN record into the next buffers.
+// in the fused pass its stale position and vacancy record into the next buffers.
 template <int MODE>
 __device__ __forceinline__ void vacancyForceM(const Geom& G, int64_t s, int p, int cx, int cy, int cz, CXPtr x,
                                               XPtr xn, float4* __restrict__ recn, XPtr f) {
@@ -290,9 +296,10 @@ __device__ __forceinline__ void forceTailM(const Geom& G, int64_t s, int p, int 
       for (int q = 0; q < full.nw[p]; q++) {
         const int64_t j = s + full.woff[p][q];
         const int8_t tj = type[j];
+        if (!wideTarget(ti, tj)) continue;
         const double e0 = xi0 - x.c[0][j], e1 = xi1 - x.c[1][j], e2 = xi2 - x.c[2][j];
         const double r2 = e0 * e0 + e1 * e1 + e2 * e2;
-        if (!isVacant(tj) && r2 < G.rc2) acc.add(e0, e1, e2, pairForce(tb, ei, species(tj), r2, dfi, rec[j].w));
+        if (r2 < G.rc2) acc.add(e0, e1, e2, pairForce(tb, ei, species(tj), r2, dfi, rec[j].w));
       }
     }
   }

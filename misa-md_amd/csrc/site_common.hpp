@@ -119,8 +119,10 @@ __device__ __forceinline__ bool leftSite(const Geom& G, int p, int cx, int cy, i
 // {ux, uy, uz, w}.  u is the FP32 displacement of the atom from its lattice
 // site (taken in FP64, then rounded), with the species in the two low
 // mantissa bits of ux (a change of at most 3 ulp); w is F'(rho) of the site,
-// written by the density pass.  A vacancy has a NaN displacement, so every
-// distance test against it fails.  Periodic images carry the same record.
+// written by the density pass.  A vacancy has a displacement of 10^4 A along
+// each axis, finite so that a term multiplied by a zero pair factor stays
+// zero, and so far away that every distance test against it fails.
+// Periodic images carry the same record.
 __device__ __forceinline__ void sitePos(const Geom& G, int p, int cx, int cy, int cz, double& X, double& Y,
                                         double& Z) {
 #pragma clang fp contract(off)
@@ -133,10 +135,8 @@ __device__ __forceinline__ float4 packRec(double u0, double u1, double u2, int s
   const uint32_t bx = (__float_as_uint(static_cast<float>(u0)) & ~3u) | static_cast<uint32_t>(sp);
   return make_float4(__uint_as_float(bx), static_cast<float>(u1), static_cast<float>(u2), w);
 }
-__device__ __forceinline__ float4 vacantRec() {
-  const float n = __uint_as_float(0x7fc00000u);
-  return make_float4(n, n, n, 0.0f);
-}
+constexpr float kVacantDisp = 1.0e4f;
+__device__ __forceinline__ float4 vacantRec() { return make_float4(kVacantDisp, kVacantDisp, kVacantDisp, 0.0f); }
 __device__ __forceinline__ int recSpecies(float ux) { return static_cast<int>(__float_as_uint(ux) & 3u); }
 // Record of the atom at x on site (p, cx, cy, cz) of species sp.
 __device__ __forceinline__ float4 siteRec(const Geom& G, int p, int cx, int cy, int cz, double x0, double x1,
@@ -317,11 +317,11 @@ __device__ __forceinline__ void addGroup3(Sum3<ACC>& acc, const int* ks, const b
         acc.flush();
         acc.chunk = t;
       }
-      if (in[u]) {
-        acc.g0 = fmaf(d0[u], fp[u], acc.g0);
-        acc.g1 = fmaf(d1[u], fp[u], acc.g1);
-        acc.g2 = fmaf(d2[u], fp[u], acc.g2);
-      }
+      // fp is 0 beyond the cutoff and the displacements are finite (a
+      // vacancy's too), so those terms add an exact zero
+      acc.g0 = fmaf(d0[u], fp[u], acc.g0);
+      acc.g1 = fmaf(d1[u], fp[u], acc.g1);
+      acc.g2 = fmaf(d2[u], fp[u], acc.g2);
     }
   } else {
 #pragma unroll
