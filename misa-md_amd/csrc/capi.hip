@@ -13,6 +13,7 @@
 #include "hook.hpp"
 #include "host_setup.hpp"
 #include "resident.hpp"
+#include "site_common.hpp"
 
 using namespace misa;
 
@@ -124,9 +125,12 @@ int buildPlaneOffsets(misa_handle* h) {
         const int64_t d = po.dp * plane + (static_cast<int64_t>(po.dz) * NY + po.dy) * NX + po.dcx;
         // lattice vector from the centre's site to the neighbour's site
         const double hdp = 0.5 * po.dp;
+        const uint32_t packed = packDelta(po.dp, po.dz, po.dy, po.dcx);
+        float packedBits;
+        std::memcpy(&packedBits, &packed, sizeof(packedBits));
         h->dlHost[p * kMaxLatOffsets + nl] =
             make_float4(static_cast<float>((po.dcx + hdp) * b.a), static_cast<float>((po.dy + hdp) * b.a),
-                        static_cast<float>((po.dz + hdp) * b.a), 0.0f);
+                        static_cast<float>((po.dz + hdp) * b.a), packedBits);
         h->lat.off[p][nl++] = static_cast<int32_t>(d);
       }
     }
@@ -1145,6 +1149,17 @@ struct misa_batch {
   hipGraphExec_t exec = nullptr;
   std::vector<uint64_t> key;  // per box: generation and position buffer the graph was captured for
   int64_t replays = 0;
+  // Boxes alike in precision, fixed step length, potential and neighbour
+  // offsets run every pass of a step as one launch for all boxes, from box
+  // records in device memory (one allocation: records, block starts, decide
+  // arguments); the graph then holds these launches instead of a branch per box.
+  bool fused = false;
+  void* fusedBlock = nullptr;
+  site::BatchBox* boxes = nullptr;
+  uint32_t* rhoStart = nullptr;
+  uint32_t* vstart = nullptr;
+  void* decideArgs = nullptr;
+  unsigned rhoBlocks = 0;
 };
 
 namespace {
@@ -1167,6 +1182,131 @@ std::vector<uint64_t> batchKey(const misa_batch* b) {
     k.push_back(static_cast<uint64_t>(h->s.cur));
   }
   return k;
+}
+
+// Can the boxes' passes run as one launch each?  Same mixed precision and
+// accumulation, fixed step length, identical potential tables and pruned
+// offsets (the lattice vectors and packed deltas), and the LDS force pass
+// available.
+bool fusable(const misa_batch* b) {
+  const misa_handle* h0 = b->hs[0];
+  if (h0->precision != MISA_PREC_MIXED || h0->s.ncu < 8) return false;
+  for (const misa_handle* h : b->hs) {
+    if (h->precision != MISA_PREC_MIXED || h->acc != h0->acc || h->max_disp > 0.0 || !h->have_pot ||
+        !h->have_offsets)
+      return false;
+    if (h->setfl.nr != h0->setfl.nr || h->setfl.nrho != h0->setfl.nrho || h->setfl.dr != h0->setfl.dr ||
+        h->setfl.drho != h0->setfl.drho || h->cF != h0->cF || h->cR != h0->cR || h->cZ != h0->cZ)
+      return false;
+    for (int t = 0; t < 3; t++)
+      if (h->species_present[t] != h0->species_present[t]) return false;
+    if (h->lat.n[0] != h0->lat.n[0] || h->lat.n[1] != h0->lat.n[1] ||
+        std::memcmp(h->dlHost, h0->dlHost, sizeof(h->dlHost)) != 0)
+      return false;
+  }
+  return true;
+}
+
+// The box records of a fused batch, with the buffers as they stand now.
+int buildFused(misa_batch* b) {
+  const int n = static_cast<int>(b->hs.size());
+  std::vector<site::BatchBox> hb(n);
+  std::vector<uint32_t> rs(n + 1, 0), vs(n + 1, 0);
+  for (int i = 0; i < n; i++) {
+    misa_handle* h = b->hs[i];
+    const DevState& s = h->s;
+    site::BatchBox& B = hb[i];
+    std::memset(&B, 0, sizeof(B));
+    B.G = s.G;
+    B.O = h->lat;
+    B.t32 = h->t32;
+    B.iv = interView(s, true);
+    B.in = s.in;
+    for (int c = 0; c < 2; c++)
+      for (int k = 0; k < 3; k++) B.x[c][k] = s.x[c][k];
+    for (int k = 0; k < 3; k++) {
+      B.v[k] = s.v[k];
+      B.f[k] = s.f[k];
+    }
+    B.type = s.type;
+    B.id = s.id;
+    B.rec[0] = s.rec[0];
+    B.rec[1] = s.rec[1];
+    B.dl = s.dl;
+    B.wmask = s.wmask;
+    B.smask = s.smask;
+    const double rcs = std::sqrt(s.G.rc2) + s.skin;
+    B.rcs2 = static_cast<float>(rcs * rcs);
+    B.skin = s.skin;
+    B.vp = verletOf(h->dt);
+    B.st = site::stageOf(s);
+    B.ctl = s.ctl;
+    B.cur = s.cur;
+    rs[i + 1] = rs[i] + site::siteGrid(s.G);
+    vs[i + 1] = vs[i] + static_cast<uint32_t>(2 * s.G.W.bpx);
+  }
+  const size_t da = decideArgsBytes();
+  auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+  const size_t oBox = 0, oRs = up(n * sizeof(site::BatchBox)), oVs = oRs + up((n + 1) * 4),
+               oDa = oVs + up((n + 1) * 4), total = oDa + up(2 * n * da);
+  std::vector<char> host(total, 0);
+  std::memcpy(host.data() + oBox, hb.data(), n * sizeof(site::BatchBox));
+  std::memcpy(host.data() + oRs, rs.data(), (n + 1) * 4);
+  std::memcpy(host.data() + oVs, vs.data(), (n + 1) * 4);
+  for (int i = 0; i < n; i++)
+    for (int f = 0; f < 2; f++) fillDecideArgs(hb[i], hb[i].cur ^ f, host.data() + oDa + (2 * i + f) * da);
+  if (b->fusedBlock) {
+    BHIP(b, hipStreamSynchronize(b->origin));
+    BHIP(b, hipFree(b->fusedBlock));
+    b->fusedBlock = nullptr;
+  }
+  char* base = nullptr;
+  BHIP(b, hipMalloc(&base, total));
+  b->fusedBlock = base;
+  BHIP(b, hipMemcpy(base, host.data(), total, hipMemcpyHostToDevice));
+  b->boxes = reinterpret_cast<site::BatchBox*>(base + oBox);
+  b->rhoStart = reinterpret_cast<uint32_t*>(base + oRs);
+  b->vstart = reinterpret_cast<uint32_t*>(base + oVs);
+  b->decideArgs = base + oDa;
+  b->rhoBlocks = rs[n];
+  return MISA_OK;
+}
+
+// Two loop bodies of every box of a fused batch as one launch per pass:
+// the inter atoms' density, the lattice density pass, the inter atoms'
+// forces, the LDS force pass with the next half step and decide staging,
+// the inter atoms' first half step and the decide (on the other buffer).
+int captureFused(misa_batch* b) {
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  const misa_handle* h0 = b->hs[0];
+  const int n = static_cast<int>(b->hs.size());
+  BHIP(b, hipStreamBeginCapture(b->origin, hipStreamCaptureModeRelaxed));
+  bool lds = true;
+  for (int f = 0; f < 2; f++) {
+    launchInterBatch(0, b->boxes, n, f, nullptr, b->origin);
+    launchRhoDfBatch(b->boxes, b->rhoStart, n, b->rhoBlocks, h0->acc, f, b->origin);
+    launchInterBatch(1, b->boxes, n, f, nullptr, b->origin);
+    lds = lds && launchForceBatch(b->boxes, b->vstart, n, h0->s.dl, h0->t32.nr, h0->s.ncu, h0->acc, f, b->origin);
+    launchInterBatch(2, b->boxes, n, f, nullptr, b->origin);
+    launchInterBatch(3, b->boxes, n, f ^ 1, b->decideArgs, b->origin);
+  }
+  const hipError_t e0 = hipGetLastError();
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(b->origin, &g);
+  if (e0 != hipSuccess) return bfail(b, MISA_E_HIP, std::string("capturing the fused step: ") + hipGetErrorString(e0));
+  if (e2 != hipSuccess) return bfail(b, MISA_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+  if (!lds) {
+    (void)hipGraphDestroy(g);
+    return bfail(b, MISA_E_HIP, "the device refused the LDS size of the fused force pass");
+  }
+  const hipError_t e3 = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e3 != hipSuccess) return bfail(b, MISA_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
+  b->key = batchKey(b);
+  return MISA_OK;
 }
 
 // Capture two loop bodies of every box, each box a branch of its own stream
@@ -1238,6 +1378,7 @@ int misa_batch_destroy(misa_batch* b) {
   (void)hipSetDevice(b->device);
   if (b->origin) (void)hipStreamSynchronize(b->origin);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->fusedBlock) (void)hipFree(b->fusedBlock);
   for (hipEvent_t e : b->done)
     if (e) (void)hipEventDestroy(e);
   if (b->fork) (void)hipEventDestroy(b->fork);
@@ -1273,7 +1414,15 @@ int misa_batch_step(misa_batch* b, int64_t nsteps, int64_t* graph_replays) {
   const int64_t pairs = (nsteps - 1) / 2;
   int rc = MISA_OK;
   if (pairs > 0) {
-    if (!b->exec || b->key != batchKey(b)) rc = captureBatch(b, plans);
+    if (!b->exec || b->key != batchKey(b)) {
+      b->fused = fusable(b);
+      if (b->fused) {
+        rc = buildFused(b);
+        if (rc == MISA_OK) rc = captureFused(b);
+      } else {
+        rc = captureBatch(b, plans);
+      }
+    }
     if (rc == MISA_OK) {
       for (size_t i = 0; i < n && rc == MISA_OK; i++) {
         if (hipEventRecord(b->done[i], b->hs[i]->stream) != hipSuccess ||

@@ -123,6 +123,14 @@ __device__ __forceinline__ void checkRho(const Ctl* ctl, double rho, double rhom
 __device__ __forceinline__ double speciesMass(int t) { return t == 0 ? 55.845 : (t == 1 ? 63.546 : 58.6934); }
 constexpr double kFtm2vDev = 1.0 / 1.0364269e-4;
 
+// Offset (target parity dp, dz, dy, dcx in cells) of a pruned neighbour packed
+// into one word, one signed byte each (in the w of the lattice-vector table),
+// so that every box of a batch can turn it into its own slot delta.
+__host__ __device__ inline uint32_t packDelta(int dp, int dz, int dy, int dcx) {
+  return (static_cast<uint32_t>(dp) & 0xffu) | ((static_cast<uint32_t>(dz) & 0xffu) << 8) |
+         ((static_cast<uint32_t>(dy) & 0xffu) << 16) | ((static_cast<uint32_t>(dcx) & 0xffu) << 24);
+}
+
 // Pruned lattice-lattice neighbour list: slot deltas per centre parity, and
 // the same deltas shifted by `lo` (the most negative delta) so that a buffer
 // load can take them as its non-negative scalar byte offset from a base that

@@ -139,4 +139,57 @@ __device__ __forceinline__ void forBucket(const InterView& iv, int64_t slot, int
   }
 }
 
+// Visits, in full-list order (the site itself first), the entries of the
+// buckets of site s (parity p) and of its full-list neighbours.  The bucket
+// heads are read eight at a time before any bucket is followed: nearly all
+// are empty, and the reads then overlap instead of queueing one round trip
+// behind the other.
+template <typename Fn>
+__device__ __forceinline__ void forNearBuckets(const InterView& iv, const FullPlaneOffsets& full, int p, int64_t s,
+                                               int ne, Fn fn) {
+  constexpr int B = 4;
+  const int n = full.n[p];
+  for (int q0 = -1; q0 < n; q0 += B) {
+    int32_t h[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int q = q0 + u;
+      h[u] = q < n ? iv.head[q < 0 ? s : s + full.off[p][q]] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      if (h[u] < 0) continue;
+      const int q = q0 + u;
+      const int64_t slot = q < 0 ? s : s + full.off[p][q];
+      for (int e = h[u]; e < ne; ++e) {
+        const uint64_t k = iv.keys[e];
+        if (static_cast<int64_t>(k >> kEntryBits) != slot) break;
+        fn(static_cast<int>(k & ((1u << kEntryBits) - 1)));
+      }
+    }
+  }
+}
+
+// Visits, in order, the full-list offsets of site s beyond the pruned list
+// that hold a pair possibly within the cutoff (the site or the target a far
+// lattice atom), calling fn(slot, type) for each; the target types are read
+// eight at a time.
+template <typename Fn>
+__device__ __forceinline__ void forWideTargets(const FullPlaneOffsets& full, const int8_t* __restrict__ type, int p,
+                                               int64_t s, int8_t ti, Fn fn) {
+  constexpr int B = 4;
+  const int n = full.nw[p];
+  for (int q0 = 0; q0 < n; q0 += B) {
+    int8_t t[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int q = q0 + u;
+      t[u] = q < n ? type[s + full.woff[p][q]] : kVacant;
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (wideTarget(ti, t[u])) fn(s + full.woff[p][q0 + u], t[u]);
+  }
+}
+
 }  // namespace misa

@@ -207,26 +207,18 @@ __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T>
     if (ti & kNearInter) {  // inter atoms whose nearest site is within the full list (atom.cpp:224-240)
       const int ne = static_cast<int>(iv.ctl->n_inter + iv.ctl->n_ghost);
       const FullPlaneOffsets& full = *iv.full;
-      auto bucket = [&](int64_t q) {
-        forBucket(iv, q, ne, [&](int e) {
-          const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
-          const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-          if (r2 < G.rc2) acc.add(rhoPair(tb, species(iv.type[iv.e_src[e]]), r2));
-        });
-      };
-      bucket(s);
-      for (int q = 0; q < full.n[p]; q++) bucket(s + full.off[p][q]);
+      forNearBuckets(iv, full, p, s, ne, [&](int e) {
+        const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        if (r2 < G.rc2) acc.add(rhoPair(tb, species(iv.type[iv.e_src[e]]), r2));
+      });
     }
     if (ti & kWide) {  // full-list offsets beyond the pruned list, near a far lattice atom
-      const FullPlaneOffsets& full = *iv.full;
-      for (int q = 0; q < full.nw[p]; q++) {
-        const int64_t j = s + full.woff[p][q];
-        const int8_t tj = type[j];
-        if (!wideTarget(ti, tj)) continue;
+      forWideTargets(*iv.full, type, p, s, ti, [&](int64_t j, int8_t tj) {
         const double d0 = xi0 - x.c[0][j], d1 = xi1 - x.c[1][j], d2 = xi2 - x.c[2][j];
         const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
         if (r2 < G.rc2) acc.add(rhoPair(tb, species(tj), r2));
-      }
+      });
     }
     rho = acc.value();
     checkRho(iv.ctl, rho, static_cast<double>(tb.rhomax));
@@ -418,27 +410,19 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
   if (ti & kNearInter) {  // forces from inter atoms on lattice neighbours (atom.cpp:397-422)
     const int ne = static_cast<int>(iv.ctl->n_inter + iv.ctl->n_ghost);
     const FullPlaneOffsets& full = *iv.full;
-    auto bucket = [&](int64_t q) {
-      forBucket(iv, q, ne, [&](int e) {
-        const int src = iv.e_src[e];
-        const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
-        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        if (r2 < G.rc2) acc.add(d0, d1, d2, pairForce(tb, ei, species(iv.type[src]), r2, dfi, dfOf<T>(iv, src)));
-      });
-    };
-    bucket(s);
-    for (int q = 0; q < full.n[p]; q++) bucket(s + full.off[p][q]);
+    forNearBuckets(iv, full, p, s, ne, [&](int e) {
+      const int src = iv.e_src[e];
+      const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
+      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      if (r2 < G.rc2) acc.add(d0, d1, d2, pairForce(tb, ei, species(iv.type[src]), r2, dfi, dfOf<T>(iv, src)));
+    });
   }
   if (ti & kWide) {  // full-list offsets beyond the pruned list, near a far lattice atom
-    const FullPlaneOffsets& full = *iv.full;
-    for (int q = 0; q < full.nw[p]; q++) {
-      const int64_t j = s + full.woff[p][q];
-      const int8_t tj = type[j];
-      if (!wideTarget(ti, tj)) continue;
+    forWideTargets(*iv.full, type, p, s, ti, [&](int64_t j, int8_t tj) {
       const double e0 = xi0 - x.c[0][j], e1 = xi1 - x.c[1][j], e2 = xi2 - x.c[2][j];
       const double r2 = e0 * e0 + e1 * e1 + e2 * e2;
       if (r2 < G.rc2) acc.add(e0, e1, e2, pairForce(tb, ei, species(tj), r2, dfi, df[j]));
-    }
+    });
   }
   double f0, f1, f2;
   sum3Value<ACC>(acc, f0, f1, f2);
@@ -460,20 +444,22 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
 // ---------------------------------------------------------------------------
 // The masks of pass 1 work as in FP64: both passes see the same records and
 // round the distances the same way (dist2M), so a skipped offset has no pair.
+// The density pass of block `blk` of one box (the kernel's own block, or a
+// block of one box of a batch).
 template <int ACC, int U>
-__global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<float> tb, CXPtr x,
-                                                  const int8_t* __restrict__ type, float4* rec,
-                                                  const float4* __restrict__ dl, double* __restrict__ rho_out,
-                                                  uint64_t* __restrict__ wmask, uint64_t* __restrict__ smask,
-                                                  float rcs2, InterView iv) {
+__device__ __forceinline__ void rhoPassM(const Geom& G, const LatOffsets& O, const Tables<float>& tb, CXPtr x,
+                                         const int8_t* __restrict__ type, float4* rec,
+                                         const float4* __restrict__ dl, double* __restrict__ rho_out,
+                                         uint64_t* __restrict__ wmask, uint64_t* __restrict__ smask, float rcs2,
+                                         const InterView& iv, uint32_t blk) {
   int64_t s;
   int p, cx, cy, cz;
-  if (!siteOfThread(G, s, p, cx, cy, cz)) return;
+  if (!siteOfBlock(G, blk, threadIdx.x, s, p, cx, cy, cz)) return;
   const int8_t ti = type[s];
   // a wavefront next to a far lattice atom (kWide) scans in full and records its masks afresh
   const bool skin =
       __builtin_amdgcn_readfirstlane(iv.ctl->skin_mode) != 0 && !__ballot(!isVacant(ti) && (ti & kWide));
-  const uint32_t wv = waveId();
+  const uint32_t wv = blk * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   Sum1<ACC> acc;
   uint64_t w0 = 0, w1 = 0;  // offsets in range for some lane (uniform among the lanes that gather)
   uint64_t s0 = 0, s1 = 0;  // offsets within cutoff + skin for some lane (full scan)
@@ -546,6 +532,28 @@ __global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<f
   rhoTailM<ACC>(G, s, p, cx, cy, cz, ti, acc, x, type, tb, iv, rec, rho_out);
 }
 
+template <int ACC, int U>
+__global__ __launch_bounds__(kBlock) void kRhoDfM(Geom G, LatOffsets O, Tables<float> tb, CXPtr x,
+                                                  const int8_t* __restrict__ type, float4* rec,
+                                                  const float4* __restrict__ dl, double* __restrict__ rho_out,
+                                                  uint64_t* __restrict__ wmask, uint64_t* __restrict__ smask,
+                                                  float rcs2, InterView iv) {
+  rhoPassM<ACC, U>(G, O, tb, x, type, rec, dl, rho_out, wmask, smask, rcs2, iv, blockIdx.x);
+}
+
+// The density pass of every box of a batch in one launch: the blocks of box
+// b are [start[b], start[b+1]), in the box's own block order (the starts are
+// multiples of 8, so a block keeps its XCD).
+template <int ACC, int U>
+__global__ __launch_bounds__(kBlock) void kRhoDfMB(const BatchBox* __restrict__ boxes, const uint32_t* start,
+                                                   int nbox, int flip) {
+  const int b = boxOfBlock(start, nbox, blockIdx.x);
+  const BatchBox& B = boxes[b];
+  const int c = B.cur ^ flip;
+  rhoPassM<ACC, U>(B.G, B.O, B.t32, cxp(B.x[c]), B.type, B.rec[c], B.dl, nullptr, B.wmask, B.smask, B.rcs2, B.iv,
+                   blockIdx.x - start[b]);
+}
+
 template <int ACC, int MODE, int U>
 __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<float> tb, CXPtr x, XPtr xn,
                                                   const int8_t* __restrict__ type, const float4* __restrict__ rec,
@@ -600,6 +608,72 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
   if (MODE == 1) waveMaxTo(db, &ctl->dispmax2);
 }
 
+// One virtual block of the LDS force pass (block b of a box): per site the
+// same pairs, in the same order and with the same rounding, as kForceM.
+// offT holds, per pruned offset of the centre's parity, the lattice vector
+// and in w what shift(w) turns into the byte shift of the neighbour's record.
+template <int ACC, int MODE, int U, typename Shift>
+__device__ __forceinline__ void forceVBlockL(const Geom& G, const LatOffsets& O, const Tables<float>& tb, CXPtr x,
+                                             XPtr xn, const int8_t* __restrict__ type,
+                                             const float4* __restrict__ rec, float4* __restrict__ recn,
+                                             const uint64_t* __restrict__ wmask, XPtr v, XPtr f,
+                                             const uint64_t* __restrict__ id, const VerletParams& vp,
+                                             const StageArgs& st, Ctl* ctl, const InterView& iv, const float4* offT,
+                                             const float4* knT, int mlo, uint32_t b, uint32_t tl, Shift shift,
+                                             unsigned long long& vb, unsigned long long& db) {
+  int64_t s;
+  int p, cx, cy, cz;
+  if (!siteOfBlock(G, b, tl, s, p, cx, cy, cz)) return;
+  const int8_t ti = type[s];
+  if (isVacant(ti)) {  // vacancy: keeps its stale position (atom.cpp:43-47)
+    vacancyForceM<MODE>(G, s, p, cx, cy, cz, x, xn, recn, f);
+    return;
+  }
+  const Rsrc rr = rsrcBelow(rec, O.lo);
+  const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairZ));
+  const Rsrc rtabD = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoD));
+  const float rc2 = static_cast<float>(G.rc2);
+  const int ei = species(ti);
+  const float4 me = rec[s];
+  const float dfi = me.w;
+  const uint32_t o16 = static_cast<uint32_t>(s) * 16u;
+  const uint32_t pairBase = static_cast<uint32_t>(ei * 3) * static_cast<uint32_t>(tb.nr);
+  const float4* T = offT + p * kMaxLatOffsets;
+  Sum3<ACC> acc;
+  const uint32_t wv = b * (kBlock / 64) + __builtin_amdgcn_readfirstlane(tl >> 6);
+  auto group = [&](auto UC, const int* ks) {
+    constexpr int V = decltype(UC)::value;
+    float4 nb[V], d[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      d[u] = T[ks[u]];
+      nb[u] = ldRec4(rr, o16 + shift(d[u].w), 0);
+    }
+    forceGroupL<ACC, V>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, knT, mlo, ks, acc);
+  };
+  forMaskGroups<U>(uniform64(wmask[2 * wv]), uniform64(wmask[2 * wv + 1]), [&](const int* ks, int nv) {
+    if (nv == U) group(std::integral_constant<int, U>{}, ks);
+    else group(std::integral_constant<int, 1>{}, ks);
+  });
+  flushChunk3(acc);
+  unsigned long long vbs = 0, dbs = 0;
+  forceTailM<ACC, MODE>(G, s, p, cx, cy, cz, ti, ei, dfi, acc, x, xn, type, rec, recn, v, f, id, vp, st, ctl, tb,
+                        iv, vbs, dbs);
+  vb = vbs > vb ? vbs : vb;
+  db = dbs > db ? dbs : db;
+}
+
+// The LDS knots of the force pass, per knot the Fe-Fe r*phi cubic and the Fe
+// density slope cubic side by side, from knot mlo on (one block fills them).
+__device__ __forceinline__ void fillKnots(const Tables<float>& tb, float4* knT, int mlo, int nkn) {
+  const float4* z4 = reinterpret_cast<const float4*>(tb.pairZ);  // row (Fe, Fe) is the first
+  const float4* s4 = reinterpret_cast<const float4*>(tb.rhoD);   // row Fe is the first
+  for (int i = threadIdx.x; i < nkn; i += kBlockP) {
+    knT[2 * i] = z4[mlo + i];
+    knT[2 * i + 1] = s4[mlo + i];
+  }
+}
+
 // The mixed force pass in persistent blocks, one per compute unit, that hold
 // in LDS the lattice vector and record shift of every pruned offset (read as
 // one broadcast per offset), the Fe-Fe r*phi cubics and the Fe density slope
@@ -620,59 +694,68 @@ __global__ __launch_bounds__(kBlockP) void kForceL(Geom G, LatOffsets O, Tables<
     d.w = __uint_as_float(k < O.n[q] ? O.sh[q][k] * 16u : 0u);
     offT[i] = d;
   }
-  const float4* z4 = reinterpret_cast<const float4*>(tb.pairZ);  // row (Fe, Fe) is the first
-  const float4* s4 = reinterpret_cast<const float4*>(tb.rhoD);   // row Fe is the first
-  for (int i = threadIdx.x; i < nkn; i += kBlockP) {
-    knT[2 * i] = z4[mlo + i];
-    knT[2 * i + 1] = s4[mlo + i];
-  }
+  fillKnots(tb, knT, mlo, nkn);
   __syncthreads();
-  const Rsrc rr = rsrcBelow(rec, O.lo);
-  const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairZ));
-  const Rsrc rtabD = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoD));
-  const float rc2 = static_cast<float>(G.rc2);
   unsigned long long vb = 0;  // |v|^2 bits after the secondstep (MODE 0), largest over the lane's sites
   unsigned long long db = 0;  // |dx|^2 bits of the drift (MODE 1), likewise
+  auto shift = [](float w) { return __float_as_uint(w); };
   forVirtualBlocks(G, [&](uint32_t b, uint32_t tl) {
-    int64_t s;
-    int p, cx, cy, cz;
-    if (!siteOfBlock(G, b, tl, s, p, cx, cy, cz)) return;
-    const int8_t ti = type[s];
-    if (isVacant(ti)) {  // vacancy: keeps its stale position (atom.cpp:43-47)
-      vacancyForceM<MODE>(G, s, p, cx, cy, cz, x, xn, recn, f);
-      return;
-    }
-    const int ei = species(ti);
-    const float4 me = rec[s];
-    const float dfi = me.w;
-    const uint32_t o16 = static_cast<uint32_t>(s) * 16u;
-    const uint32_t pairBase = static_cast<uint32_t>(ei * 3) * static_cast<uint32_t>(tb.nr);
-    const float4* T = offT + p * kMaxLatOffsets;
-    Sum3<ACC> acc;
-    const uint32_t wv = b * (kBlock / 64) + __builtin_amdgcn_readfirstlane(tl >> 6);
-    auto group = [&](auto UC, const int* ks) {
-      constexpr int V = decltype(UC)::value;
-      float4 nb[V], d[V];
-#pragma unroll
-      for (int u = 0; u < V; ++u) {
-        d[u] = T[ks[u]];
-        nb[u] = ldRec4(rr, o16 + __float_as_uint(d[u].w), 0);
-      }
-      forceGroupL<ACC, V>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, knT, mlo, ks, acc);
-    };
-    forMaskGroups<U>(uniform64(wmask[2 * wv]), uniform64(wmask[2 * wv + 1]), [&](const int* ks, int nv) {
-      if (nv == U) group(std::integral_constant<int, U>{}, ks);
-      else group(std::integral_constant<int, 1>{}, ks);
-    });
-    flushChunk3(acc);
-    unsigned long long vbs = 0, dbs = 0;
-    forceTailM<ACC, MODE>(G, s, p, cx, cy, cz, ti, ei, dfi, acc, x, xn, type, rec, recn, v, f, id, vp, st, ctl, tb,
-                          iv, vbs, dbs);
-    vb = vbs > vb ? vbs : vb;
-    db = dbs > db ? dbs : db;
+    forceVBlockL<ACC, MODE, U>(G, O, tb, x, xn, type, rec, recn, wmask, v, f, id, vp, st, ctl, iv, offT, knT, mlo, b,
+                               tl, shift, vb, db);
   });
   if (MODE == 0 && vp.vmax) waveMaxTo(vb, vp.vmax);
   if (MODE == 1) waveMaxTo(db, &ctl->dispmax2);
+}
+
+// The LDS force pass of every box of a batch in one persistent launch: on
+// each XCD the blocks walk the virtual blocks of box 0, then box 1, ...
+// (vstart[b] counts the virtual blocks per XCD before box b).  All boxes
+// share the potential (the knots in LDS) and the pruned offsets (lattice
+// vectors and packed deltas in dlp, 2 x kMaxLatOffsets); each turns a
+// packed delta into its own record shift.
+template <int ACC, int MODE, int U>
+__global__ __launch_bounds__(kBlockP) void kForceLB(const BatchBox* __restrict__ boxes, const uint32_t* vstart,
+                                                    int nbox, int flip, const float4* __restrict__ dlp, int mlo,
+                                                    int nkn) {
+  __shared__ float4 offT[2 * kMaxLatOffsets];
+  extern __shared__ float4 knT[];
+  for (int i = threadIdx.x; i < 2 * kMaxLatOffsets; i += kBlockP) offT[i] = dlp[i];
+  fillKnots(boxes[0].t32, knT, mlo, nkn);
+  __syncthreads();
+  constexpr uint32_t V = kBlockP / kBlock;
+  const uint32_t c = blockIdx.x & 7u, q = blockIdx.x >> 3, Q = gridDim.x >> 3;
+  const uint32_t sub = __builtin_amdgcn_readfirstlane(threadIdx.x / kBlock);
+  const uint32_t tl = threadIdx.x % kBlock;
+  const uint32_t nvb = vstart[nbox];
+  unsigned long long vb = 0, db = 0;
+  int lastBox = -1;
+  for (uint32_t k = q * V + sub; k < nvb; k += Q * V) {
+    const int bi = boxOfBlock(vstart, nbox, k);
+    const BatchBox& B = boxes[bi];
+    if (bi != lastBox) {  // the maxima go to the box they came from
+      if (lastBox >= 0) {
+        if (MODE == 0 && boxes[lastBox].vp.vmax) waveMaxTo(vb, boxes[lastBox].vp.vmax);
+        if (MODE == 1) waveMaxTo(db, &boxes[lastBox].ctl->dispmax2);
+      }
+      vb = db = 0;
+      lastBox = bi;
+    }
+    const int cb = B.cur ^ flip;
+    const int32_t plane = static_cast<int32_t>(B.G.plane), NX = B.G.NX, NY = B.G.NY, lo = B.O.lo;
+    auto shift = [&](float w) {
+      const uint32_t u = __builtin_amdgcn_readfirstlane(__float_as_uint(w));
+      const int32_t dp = static_cast<int8_t>(u & 0xffu), dz = static_cast<int8_t>((u >> 8) & 0xffu);
+      const int32_t dy = static_cast<int8_t>((u >> 16) & 0xffu), dcx = static_cast<int8_t>(u >> 24);
+      return static_cast<uint32_t>(dp * plane + (dz * NY + dy) * NX + dcx - lo) * 16u;
+    };
+    forceVBlockL<ACC, MODE, U>(B.G, B.O, B.t32, cxp(B.x[cb]), xp(B.x[1 - cb]), B.type, B.rec[cb], B.rec[1 - cb],
+                               B.wmask, xp(B.v), xp(B.f), B.id, B.vp, B.st, B.ctl, B.iv, offT, knT, mlo,
+                               c + 8u * (k - vstart[bi]), tl, shift, vb, db);
+  }
+  if (lastBox >= 0) {
+    if (MODE == 0 && boxes[lastBox].vp.vmax) waveMaxTo(vb, boxes[lastBox].vp.vmax);
+    if (MODE == 1) waveMaxTo(db, &boxes[lastBox].ctl->dispmax2);
+  }
 }
 
 // LDS knots of the force pass: the Fe-Fe r*phi and Fe slope cubics (32 bytes
@@ -795,6 +878,30 @@ int launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t6
 #undef MISA_FORCE_M
 #undef MISA_FORCE_MU
   return kind;
+}
+
+void launchRhoDfBatch(const BatchBox* boxes, const uint32_t* rhoStart, int nbox, unsigned blocks, int acc,
+                      int flip, hipStream_t st) {
+  if (acc == 0) kRhoDfMB<0, 4><<<blocks, kBlock, 0, st>>>(boxes, rhoStart, nbox, flip);
+  else if (acc == 1) kRhoDfMB<1, 4><<<blocks, kBlock, 0, st>>>(boxes, rhoStart, nbox, flip);
+  else kRhoDfMB<2, 4><<<blocks, kBlock, 0, st>>>(boxes, rhoStart, nbox, flip);
+}
+
+bool launchForceBatch(const BatchBox* boxes, const uint32_t* vstart, int nbox, const float4* dl, int nr, int ncu,
+                      int acc, int flip, hipStream_t st) {
+  const LdsKnots lk = ldsKnotsForce(nr);
+  const unsigned grid = static_cast<unsigned>(ncu / 8 * 8);
+#define MISA_FORCE_B(A)                                                                                      \
+  do {                                                                                                       \
+    static const bool ready = setLdsBytes(reinterpret_cast<const void*>(&kForceLB<A, 1, 4>), kForceLdsBytes); \
+    if (!ready) return false;                                                                                \
+    kForceLB<A, 1, 4><<<grid, kBlockP, lk.bytes, st>>>(boxes, vstart, nbox, flip, dl, lk.mlo, lk.nkn);        \
+  } while (0)
+  if (acc == 0) MISA_FORCE_B(0);
+  else if (acc == 1) MISA_FORCE_B(1);
+  else MISA_FORCE_B(2);
+#undef MISA_FORCE_B
+  return true;
 }
 
 }  // namespace misa

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <cstring>
 
 #include "device_common.hpp"
 #include "inter_common.hpp"
@@ -239,13 +240,12 @@ constexpr int kFlagBlocks = 1024;
 // and the flags it set, kNearInter around its entries and kWide around its
 // far lattice atoms and their images (atom.cpp:21-84 rebuilds the index each
 // step; inter_atom_list.cpp:27-45).
-__global__ __launch_bounds__(kBlock) void kUnflag(Geom G, InterDev in, int8_t* ltype, const Ctl* ctl) {
+__device__ __forceinline__ void unflagBody(const Geom& G, const InterDev& in, int8_t* ltype, const Ctl* ctl,
+                                           int64_t t0, int64_t stride) {
   const int nprev = static_cast<int>(ctl->n_prev);
   const int nfarPrev = static_cast<int>(ctl->n_far_prev);
   const FullPlaneOffsets& full = *in.full;
   const int nfull = 1 + max(full.n[0], full.n[1]);
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
-  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   for (int64_t q = t0; q < nprev; q += stride) in.head[in.prev_slot[q]] = -1;
   for (int64_t q = t0; q < static_cast<int64_t>(nprev) * nfull; q += stride) {
     const int e = static_cast<int>(q / nfull), k = static_cast<int>(q % nfull) - 1;
@@ -268,11 +268,15 @@ __global__ __launch_bounds__(kBlock) void kUnflag(Geom G, InterDev in, int8_t* l
       if (!isVacant(v) && (v & kFar)) typeBitsAnd(ltype, slot, static_cast<int8_t>(~kFar));
     });
 }
+__global__ __launch_bounds__(kBlock) void kUnflag(Geom G, InterDev in, int8_t* ltype, const Ctl* ctl) {
+  unflagBody(G, in, ltype, ctl, static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x,
+             static_cast<int64_t>(gridDim.x) * kBlock);
+}
 
 // Decide, step 2: the list work in one workgroup (the lists hold the
 // interstitials and the atoms that moved more than 0.2a this step: hundreds
 // to thousands in a developed cascade).
-__global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
+__device__ __forceinline__ void decideBody(const DecideArgs& A) {
   const Geom& G = A.G;
   InterDev in = A.in;
   const int tid = threadIdx.x;
@@ -495,19 +499,19 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) {
     c->rescan = 0;
   }
 }
+__global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) { decideBody(A); }
 
 // Decide, step 3: flag the sub-box sites within the full neighbour list of
 // every entry (kNearInter: their sums visit the inter atoms), and those
 // within the full neighbourhood of every lattice atom farther than 0.2a from
 // its site and of its periodic images (kWide: their sums visit the whole full
 // list); the far list becomes the one the next decide clears.
-__global__ __launch_bounds__(kBlock) void kFlag(Geom G, InterDev in, int8_t* ltype, const Ctl* ctl) {
+__device__ __forceinline__ void flagBody(const Geom& G, const InterDev& in, int8_t* ltype, const Ctl* ctl,
+                                         int64_t t0, int64_t stride) {
   const int ne = static_cast<int>(ctl->n_prev);
   const int nfar = static_cast<int>(ctl->n_far_prev);
   const FullPlaneOffsets& full = *in.full;
   const int nfull = 1 + max(full.n[0], full.n[1]);
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
-  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   for (int64_t q = t0; q < static_cast<int64_t>(ne) * nfull; q += stride) {
     const int e = static_cast<int>(q / nfull), k = static_cast<int>(q % nfull) - 1;
     fullNeighbour(G, full, in.e_slot[e], k, [&](int64_t t) {
@@ -530,6 +534,10 @@ __global__ __launch_bounds__(kBlock) void kFlag(Geom G, InterDev in, int8_t* lty
     });
     in.far_prev[q] = in.far_new[q];
   }
+}
+__global__ __launch_bounds__(kBlock) void kFlag(Geom G, InterDev in, int8_t* ltype, const Ctl* ctl) {
+  flagBody(G, in, ltype, ctl, static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x,
+           static_cast<int64_t>(gridDim.x) * kBlock);
 }
 
 __global__ void kRescan(Ctl* ctl) { ctl->rescan = 1; }
@@ -566,14 +574,14 @@ __device__ __forceinline__ double waveSum(double v) {
 // atom::interRho (atom.cpp:194-284): lattice (nearest site, full list), same
 // bucket without itself, then the buckets of the full list; F' at once.
 template <typename T>
-__global__ __launch_bounds__(kIT) void kInterRho(Geom G, Tables<T> tb, InterView iv, CXPtr lx,
-                                                 const int8_t* __restrict__ ltype, InterDev in) {
+__device__ __forceinline__ void interRhoBody(const Geom& G, const Tables<T>& tb, const InterView& iv, CXPtr lx,
+                                             const int8_t* __restrict__ ltype, const InterDev& in, int w0,
+                                             int wstride) {
   const int nl = static_cast<int>(iv.ctl->n_inter);
   const int ne = nl + static_cast<int>(iv.ctl->n_ghost);
   const FullPlaneOffsets& full = *iv.full;
   const int lane = threadIdx.x & 63;
-  const int w0 = blockIdx.x * (kIT / 64) + static_cast<int>(threadIdx.x >> 6);
-  for (int i = w0; i < nl; i += kWaveBlocks * (kIT / 64)) {
+  for (int i = w0; i < nl; i += wstride) {
     const int64_t n = in.e_slot[i];
     if (!isSubBoxSlot(G, n)) continue;
     const double xi0 = in.x[0][i], xi1 = in.x[1][i], xi2 = in.x[2][i];
@@ -608,18 +616,24 @@ __global__ __launch_bounds__(kIT) void kInterRho(Geom G, Tables<T> tb, InterView
     }
   }
 }
+template <typename T>
+__global__ __launch_bounds__(kIT) void kInterRho(Geom G, Tables<T> tb, InterView iv, CXPtr lx,
+                                                 const int8_t* __restrict__ ltype, InterDev in) {
+  interRhoBody<T>(G, tb, iv, lx, ltype, in, blockIdx.x * (kIT / 64) + static_cast<int>(threadIdx.x >> 6),
+                  kWaveBlocks * (kIT / 64));
+}
 
 // atom::interForce (atom.cpp:360-473) + secondstep for the inter list.
 template <typename T, int MODE>
-__global__ __launch_bounds__(kIT) void kInterForce(Geom G, Tables<T> tb, InterView iv, CXPtr lx,
-                                                   const int8_t* __restrict__ ltype, const T* __restrict__ ldf,
-                                                   int ldfStride, InterDev in, VerletParams vp) {
+__device__ __forceinline__ void interForceBody(const Geom& G, const Tables<T>& tb, const InterView& iv, CXPtr lx,
+                                               const int8_t* __restrict__ ltype, const T* __restrict__ ldf,
+                                               int ldfStride, const InterDev& in, const VerletParams& vp, int w0,
+                                               int wstride) {
   const int nl = static_cast<int>(iv.ctl->n_inter);
   const int ne = nl + static_cast<int>(iv.ctl->n_ghost);
   const FullPlaneOffsets& full = *iv.full;
   const int lane = threadIdx.x & 63;
-  const int w0 = blockIdx.x * (kIT / 64) + static_cast<int>(threadIdx.x >> 6);
-  for (int i = w0; i < nl; i += kWaveBlocks * (kIT / 64)) {
+  for (int i = w0; i < nl; i += wstride) {
     const int64_t n = in.e_slot[i];
     double f0 = 0, f1 = 0, f2 = 0;
     const int ei = species(in.type[i]);
@@ -674,12 +688,20 @@ __global__ __launch_bounds__(kIT) void kInterForce(Geom G, Tables<T> tb, InterVi
     }
   }
 }
+template <typename T, int MODE>
+__global__ __launch_bounds__(kIT) void kInterForce(Geom G, Tables<T> tb, InterView iv, CXPtr lx,
+                                                   const int8_t* __restrict__ ltype, const T* __restrict__ ldf,
+                                                   int ldfStride, InterDev in, VerletParams vp) {
+  interForceBody<T, MODE>(G, tb, iv, lx, ltype, ldf, ldfStride, in, vp,
+                          blockIdx.x * (kIT / 64) + static_cast<int>(threadIdx.x >> 6), kWaveBlocks * (kIT / 64));
+}
 
 // NewtonMotion::firststep for the inter list (newton_motion.cpp:46-54).
-__global__ __launch_bounds__(kIT) void kInterFirst(InterDev in, const Ctl* ctl, VerletParams vp) {
+__device__ __forceinline__ void interFirstBody(const InterDev& in, const Ctl* ctl, const VerletParams& vp, int i0,
+                                               int istride) {
 #pragma clang fp contract(off)  // the half kick and drift round as newton_motion.cpp does
   const int nl = static_cast<int>(ctl->n_inter);
-  for (int i = blockIdx.x * kIT + threadIdx.x; i < nl; i += kIT * kIB) {
+  for (int i = i0; i < nl; i += istride) {
     const StepDt sd = stepDt(vp, species(in.type[i]));
     const double c = sd.c;
     for (int d = 0; d < 3; d++) {
@@ -688,6 +710,9 @@ __global__ __launch_bounds__(kIT) void kInterFirst(InterDev in, const Ctl* ctl, 
       in.x[d][i] += sd.dt * v;
     }
   }
+}
+__global__ __launch_bounds__(kIT) void kInterFirst(InterDev in, const Ctl* ctl, VerletParams vp) {
+  interFirstBody(in, ctl, vp, blockIdx.x * kIT + threadIdx.x, kIT * kIB);
 }
 
 __global__ __launch_bounds__(kIT) void kInterUpload(InterDev in, const AtomElement* __restrict__ a, int64_t n,
@@ -895,7 +920,83 @@ __global__ void kDtUpdate(Ctl* ctl, double dt_max, double max_disp) {
   ctl->vmax2 = 0ull;
 }
 
+// ---------------------------------------------------------------------------
+// a batch of boxes: each kernel above once for all boxes (kBatchBlocks
+// blocks per box, or one workgroup per box for the list work)
+// ---------------------------------------------------------------------------
+constexpr int kBatchBlocks = 8;
+
+__global__ __launch_bounds__(kBlock) void kUnflagB(const BatchBox* __restrict__ boxes) {
+  const BatchBox& B = boxes[blockIdx.x / kBatchBlocks];
+  unflagBody(B.G, B.in, B.type, B.ctl, static_cast<int64_t>(blockIdx.x % kBatchBlocks) * kBlock + threadIdx.x,
+             static_cast<int64_t>(kBatchBlocks) * kBlock);
+}
+__global__ __launch_bounds__(kBlock) void kFlagB(const BatchBox* __restrict__ boxes) {
+  const BatchBox& B = boxes[blockIdx.x / kBatchBlocks];
+  flagBody(B.G, B.in, B.type, B.ctl, static_cast<int64_t>(blockIdx.x % kBatchBlocks) * kBlock + threadIdx.x,
+           static_cast<int64_t>(kBatchBlocks) * kBlock);
+}
+// args[2 b + flip]: the decide arguments of box b for either buffer
+__global__ __launch_bounds__(kDT) void kDecideB(const DecideArgs* __restrict__ args, int flip) {
+  decideBody(args[2 * blockIdx.x + flip]);
+}
+__global__ __launch_bounds__(kIT) void kInterRhoB(const BatchBox* __restrict__ boxes, int flip) {
+  const BatchBox& B = boxes[blockIdx.x / kBatchBlocks];
+  const int c = B.cur ^ flip;
+  interRhoBody<float>(B.G, B.t32, B.iv, cxp(B.x[c]), B.type, B.in,
+                      static_cast<int>(blockIdx.x % kBatchBlocks) * (kIT / 64) + static_cast<int>(threadIdx.x >> 6),
+                      kBatchBlocks * (kIT / 64));
+}
+__global__ __launch_bounds__(kIT) void kInterForceB(const BatchBox* __restrict__ boxes, int flip) {
+  const BatchBox& B = boxes[blockIdx.x / kBatchBlocks];
+  const int c = B.cur ^ flip;
+  const float* ldf = reinterpret_cast<const float*>(B.rec[c]) + 3;  // lattice F' is the w of the records
+  interForceBody<float, 0>(B.G, B.t32, B.iv, cxp(B.x[c]), B.type, ldf, 4, B.in, B.vp,
+                           static_cast<int>(blockIdx.x % kBatchBlocks) * (kIT / 64) +
+                               static_cast<int>(threadIdx.x >> 6),
+                           kBatchBlocks * (kIT / 64));
+}
+__global__ __launch_bounds__(kIT) void kInterFirstB(const BatchBox* __restrict__ boxes) {
+  const BatchBox& B = boxes[blockIdx.x / kBatchBlocks];
+  interFirstBody(B.in, B.ctl, B.vp, static_cast<int>(blockIdx.x % kBatchBlocks) * kIT + threadIdx.x,
+                 kBatchBlocks * kIT);
+}
+
 }  // namespace
+
+size_t decideArgsBytes() { return sizeof(DecideArgs); }
+
+void fillDecideArgs(const BatchBox& b, int c, void* out) {
+  DecideArgs A;
+  A.G = b.G;
+  A.mode = 0;
+  A.cur = c;
+  for (int k = 0; k < 3; k++) {
+    A.lx[k] = b.x[c][k];
+    A.lv[k] = b.v[k];
+  }
+  A.ltype = b.type;
+  A.lid = b.id;
+  A.lrec = b.rec[c];
+  A.in = b.in;
+  A.ctl = b.ctl;
+  A.skin = b.skin;
+  std::memcpy(out, &A, sizeof(A));
+}
+
+void launchInterBatch(int which, const BatchBox* boxes, int nbox, int flip, const void* decideArgs, hipStream_t st) {
+  const unsigned g = static_cast<unsigned>(nbox * kBatchBlocks);
+  switch (which) {
+    case 0: kInterRhoB<<<g, kIT, 0, st>>>(boxes, flip); break;
+    case 1: kInterForceB<<<g, kIT, 0, st>>>(boxes, flip); break;
+    case 2: kInterFirstB<<<g, kIT, 0, st>>>(boxes); break;
+    default:  // the decide of every box, on buffer cur ^ flip
+      kUnflagB<<<g, kBlock, 0, st>>>(boxes);
+      kDecideB<<<static_cast<unsigned>(nbox), kDT, 0, st>>>(static_cast<const DecideArgs*>(decideArgs), flip);
+      kFlagB<<<g, kBlock, 0, st>>>(boxes);
+      break;
+  }
+}
 
 void launchAdaptiveDt(const DevState& s, double dt_max, double max_disp, bool sites, hipStream_t st) {
   if (sites) kVmaxSites<<<kVmaxBlocks, kBlock, 0, st>>>(s.G, cxp(s.v), s.type, s.ctl);

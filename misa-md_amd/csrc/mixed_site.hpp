@@ -204,25 +204,18 @@ __device__ __forceinline__ void rhoTailM(const Geom& G, int64_t s, int p, int cx
       const FullPlaneOffsets& full = *iv.full;
       if (ti & kNearInter) {  // inter atoms whose nearest site is within the full list (atom.cpp:224-240)
         const int ne = static_cast<int>(iv.ctl->n_inter + iv.ctl->n_ghost);
-        auto bucket = [&](int64_t q) {
-          forBucket(iv, q, ne, [&](int e) {
-            const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
-            const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-            if (r2 < G.rc2) acc.add(rhoPair(tb, species(iv.type[iv.e_src[e]]), r2));
-          });
-        };
-        bucket(s);
-        for (int q = 0; q < full.n[p]; q++) bucket(s + full.off[p][q]);
+        forNearBuckets(iv, full, p, s, ne, [&](int e) {
+          const double d0 = xi0 - iv.e_x[0][e], d1 = xi1 - iv.e_x[1][e], d2 = xi2 - iv.e_x[2][e];
+          const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+          if (r2 < G.rc2) acc.add(rhoPair(tb, species(iv.type[iv.e_src[e]]), r2));
+        });
       }
       if (ti & kWide) {  // full-list offsets beyond the pruned list, near a far lattice atom
-        for (int q = 0; q < full.nw[p]; q++) {
-          const int64_t j = s + full.woff[p][q];
-          const int8_t tj = type[j];
-          if (!wideTarget(ti, tj)) continue;
+        forWideTargets(full, type, p, s, ti, [&](int64_t j, int8_t tj) {
           const double d0 = xi0 - x.c[0][j], d1 = xi1 - x.c[1][j], d2 = xi2 - x.c[2][j];
           const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
           if (r2 < G.rc2) acc.add(rhoPair(tb, species(tj), r2));
-        }
+        });
       }
     }
     rho = acc.value();
@@ -281,26 +274,19 @@ __device__ __forceinline__ void forceTailM(const Geom& G, int64_t s, int p, int 
     const FullPlaneOffsets& full = *iv.full;
     if (ti & kNearInter) {  // forces from inter atoms on lattice neighbours (atom.cpp:397-422)
       const int ne = static_cast<int>(iv.ctl->n_inter + iv.ctl->n_ghost);
-      auto bucket = [&](int64_t q) {
-        forBucket(iv, q, ne, [&](int e) {
-          const int src = iv.e_src[e];
-          const double e0 = xi0 - iv.e_x[0][e], e1 = xi1 - iv.e_x[1][e], e2 = xi2 - iv.e_x[2][e];
-          const double r2 = e0 * e0 + e1 * e1 + e2 * e2;
-          if (r2 < G.rc2) acc.add(e0, e1, e2, pairForce(tb, ei, species(iv.type[src]), r2, dfi, iv.df32[src]));
-        });
-      };
-      bucket(s);
-      for (int q = 0; q < full.n[p]; q++) bucket(s + full.off[p][q]);
+      forNearBuckets(iv, full, p, s, ne, [&](int e) {
+        const int src = iv.e_src[e];
+        const double e0 = xi0 - iv.e_x[0][e], e1 = xi1 - iv.e_x[1][e], e2 = xi2 - iv.e_x[2][e];
+        const double r2 = e0 * e0 + e1 * e1 + e2 * e2;
+        if (r2 < G.rc2) acc.add(e0, e1, e2, pairForce(tb, ei, species(iv.type[src]), r2, dfi, iv.df32[src]));
+      });
     }
     if (ti & kWide) {  // full-list offsets beyond the pruned list, near a far lattice atom
-      for (int q = 0; q < full.nw[p]; q++) {
-        const int64_t j = s + full.woff[p][q];
-        const int8_t tj = type[j];
-        if (!wideTarget(ti, tj)) continue;
+      forWideTargets(full, type, p, s, ti, [&](int64_t j, int8_t tj) {
         const double e0 = xi0 - x.c[0][j], e1 = xi1 - x.c[1][j], e2 = xi2 - x.c[2][j];
         const double r2 = e0 * e0 + e1 * e1 + e2 * e2;
         if (r2 < G.rc2) acc.add(e0, e1, e2, pairForce(tb, ei, species(tj), r2, dfi, rec[j].w));
-      }
+      });
     }
   }
   double f0, f1, f2;

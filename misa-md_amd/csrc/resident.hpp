@@ -154,4 +154,27 @@ int64_t launchInterThermo(const DevState& s, hipStream_t st);
 int64_t launchInterEnergy(const DevState& s, const Tables<double>& t64, hipStream_t st);
 void launchSetv(const DevState& s, int64_t slot, double dv0, double dv1, double dv2, hipStream_t st);
 
+// Batches whose step passes run as one launch for all boxes (the boxes'
+// records, site_common.hpp BatchBox, in device memory).
+namespace site {
+struct BatchBox;
+}
+// Inter-atom work of every box: 0 density and F', 1 force and half kick,
+// 2 first half step, 3 the decide (clear flags, list work, set flags) on
+// buffer cur ^ flip, with decideArgs holding two records per box (for
+// either buffer) of decideArgsBytes() each, filled by fillDecideArgs.
+size_t decideArgsBytes();
+void fillDecideArgs(const site::BatchBox& b, int cur, void* out);
+void launchInterBatch(int which, const site::BatchBox* boxes, int nbox, int flip, const void* decideArgs,
+                      hipStream_t st);
+// The mixed density pass of every box (rhoStart: blocks per box, prefix
+// sums) and the LDS force pass with the next half step and decide staging
+// (MODE 1; vstart: virtual blocks per XCD, prefix sums; dl: the shared
+// lattice vectors and packed deltas).  false when the device refuses the
+// force pass's LDS size.
+void launchRhoDfBatch(const site::BatchBox* boxes, const uint32_t* rhoStart, int nbox, unsigned blocks, int acc,
+                      int flip, hipStream_t st);
+bool launchForceBatch(const site::BatchBox* boxes, const uint32_t* vstart, int nbox, const float4* dl, int nr,
+                      int ncu, int acc, int flip, hipStream_t st);
+
 }  // namespace misa

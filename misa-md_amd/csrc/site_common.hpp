@@ -355,8 +355,46 @@ __device__ __forceinline__ void sum3Value<1>(const Sum3<1>& s, double& f0, doubl
   f2 = s.c.value();
 }
 
-inline XPtr xp(double* const (&a)[3]) { return XPtr{{a[0], a[1], a[2]}}; }
-inline CXPtr cxp(double* const (&a)[3]) { return CXPtr{{a[0], a[1], a[2]}}; }
+__host__ __device__ inline XPtr xp(double* const (&a)[3]) { return XPtr{{a[0], a[1], a[2]}}; }
+__host__ __device__ inline CXPtr cxp(double* const (&a)[3]) { return CXPtr{{a[0], a[1], a[2]}}; }
+
+// One box of a batch whose passes run as one launch for all boxes: what the
+// single-box launches of the step take as arguments, with the position and
+// record buffers of both parities of the step (cur is the buffer the first
+// captured step reads; every step flips it).
+struct BatchBox {
+  Geom G;
+  LatOffsets O;
+  Tables<float> t32;
+  InterView iv;
+  InterDev in;
+  double* x[2][3];
+  double* v[3];
+  double* f[3];
+  int8_t* type;
+  uint64_t* id;
+  float4* rec[2];
+  const float4* dl;
+  uint64_t* wmask;
+  uint64_t* smask;
+  float rcs2;
+  double skin;
+  VerletParams vp;
+  StageArgs st;
+  Ctl* ctl;
+  int cur;
+};
+
+// The box of block `blk` when box b owns blocks [start[b], start[b+1]).
+__device__ __forceinline__ int boxOfBlock(const uint32_t* start, int nbox, uint32_t blk) {
+  int lo = 0, hi = nbox - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 inline StageArgs stageOf(const DevState& s) {
   StageArgs a;
