@@ -260,6 +260,22 @@ int misa_get_launch_shape(const misa_handle* h, int32_t* ty, int32_t* gathers_rh
 #define MISA_LDS_ON 1
 #define MISA_LDS_OFF 2
 int misa_set_lds_tables(misa_handle* h, int32_t mode);
+/* Work-balanced XCD slabs (default on; no counterpart in the reference, a
+ * placement choice of this build's kernels).  The site passes give each of
+ * the device's 8 XCDs a z-slab of the box; with balancing on, the first
+ * mixed LDS force pass of every misa_step call measures the cycles spent per
+ * z-plane and, after the call, the slabs are re-cut so that each holds about
+ * an eighth of that work (a cascade's disordered core otherwise keeps one
+ * XCD busy while the others wait).  Off returns to the even cut.  Every site
+ * sees the same pairs in the same order either way, so the trajectory does
+ * not change. */
+int misa_set_xcd_balance(misa_handle* h, int32_t on);
+/* The slab bounds: XCD c walks planes [zlo[c], zlo[c+1]) of the box. */
+int misa_xcd_slabs(const misa_handle* h, int32_t zlo[9]);
+/* Set the slab bounds (zlo[0] = 0, zlo[8] = planes of the box,
+ * non-decreasing, no slab above twice an eighth of the box's sites; an empty
+ * slab leaves its XCD idle) and turn balancing off. */
+int misa_set_xcd_slabs(misa_handle* h, const int32_t zlo[9]);
 /* The force pass the last force evaluation launched (-1 before any):
  * kForce (FP64), kForceM (mixed, spline segments from global memory) or
  * kForceL (mixed, persistent blocks with the majority species' segments in

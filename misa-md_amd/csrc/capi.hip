@@ -60,6 +60,10 @@ struct misa_handle {
   int force_pass = -1;       // MISA_FORCE_PASS_* of the last force pass launched
   bool range_check = false;  // a density beyond the F(rho) table fails the call
   int64_t flags_seen = 0;    // the device flags at the last check
+  // XCD slabs cut by measured work (misa_set_xcd_balance); virtual blocks
+  // per parity per XCD the wave-mask and partial-sum arrays are sized for
+  bool balance = true;
+  int64_t bpx_cap = 0;
 };
 
 namespace {
@@ -162,6 +166,24 @@ int buildPlaneOffsets(misa_handle* h) {
 // Traversal of the site kernels: z-slab per XCD, y-tiles of up to 32 rows
 // (on the 200^3 cascade box, tiles of 20-25 rows step about 1 % faster than
 // tiles of 10, and tiles of 40 rows or more are slower again).
+// Slab bounds cut at whole planes, and the virtual blocks per parity of the
+// largest slab.
+void setSlabs(Geom& G, const int* zlo) {
+  Walk& W = G.W;
+  int64_t bmax = 0;
+  for (int c = 0; c <= 8; c++) W.zlo[c] = zlo[c];
+  for (int c = 0; c < 8; c++) {
+    const int64_t sites = static_cast<int64_t>(W.zlo[c + 1] - W.zlo[c]) * G.n1 * G.n0;
+    bmax = std::max(bmax, (sites + 255) / 256);
+  }
+  W.bpx = bmax;
+}
+void evenSlabs(Geom& G) {
+  int zlo[9];
+  for (int c = 0; c <= 8; c++) zlo[c] = static_cast<int>((static_cast<int64_t>(c) * G.n2) / 8);
+  setSlabs(G, zlo);
+}
+
 void makeWalk(Geom& G) {
   Walk& W = G.W;
   W.ty = 1;
@@ -170,13 +192,7 @@ void makeWalk(Geom& G) {
       W.ty = t;
       break;
     }
-  int64_t bmax = 0;
-  for (int c = 0; c <= 8; c++) W.zlo[c] = static_cast<int>((static_cast<int64_t>(c) * G.n2) / 8);
-  for (int c = 0; c < 8; c++) {
-    const int64_t sites = static_cast<int64_t>(W.zlo[c + 1] - W.zlo[c]) * G.n1 * G.n0;
-    bmax = std::max(bmax, (sites + 255) / 256);
-  }
-  W.bpx = bmax;
+  evenSlabs(G);
 }
 
 int syncAndCheck(misa_handle* h) {
@@ -376,8 +392,11 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
   if (rc == MISA_OK) rc |= devAlloc(h, &s.rec[0], slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.rec[1], slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.dl, 2 * kMaxLatOffsets);
-  if (rc == MISA_OK) rc |= devAlloc(h, &s.wmask, static_cast<size_t>(2 * 4 * 16 * G.W.bpx));
-  if (rc == MISA_OK) rc |= devAlloc(h, &s.smask, static_cast<size_t>(2 * 4 * 16 * G.W.bpx));
+  // a balanced slab may hold up to twice the planes of an even cut
+  h->bpx_cap = 2 * G.W.bpx;
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.wmask, static_cast<size_t>(2 * 4 * 16 * h->bpx_cap));
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.smask, static_cast<size_t>(2 * 4 * 16 * h->bpx_cap));
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.zwork, static_cast<size_t>(G.n2));
   if (rc == MISA_OK) rc |= devAlloc(h, &s.type, slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.id, slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.ctl, 1);
@@ -385,7 +404,7 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess) s.ncu = ncu;
   }
-  const int64_t blocks = 16 * G.W.bpx;
+  const int64_t blocks = 16 * h->bpx_cap;
   // room for the site kernels' 7 partials per block and the inter kernels' 5
   s.npartials = std::max<int64_t>(8 * blocks, 8 * kInterBlocks);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.partials, static_cast<size_t>(s.npartials));
@@ -975,6 +994,48 @@ int finishSteps(misa_handle* h, int64_t nsteps, hipEvent_t e0, hipEvent_t e1) {
   return rc;
 }
 
+// Re-cut the XCD z-slabs so that each holds about an eighth of the work the
+// LDS force pass measured per plane (a cascade's disordered core costs
+// several times a perfect lattice's planes, and one slab holding it keeps
+// the other XCDs waiting).  Each slab keeps at least one plane and at most
+// the planes the wave-mask arrays are sized for; a cut that lowers the
+// busiest slab's work by less than 3 % is not taken.  A new cut bumps the
+// generation, so the next step scans the density pass's offsets in full.
+int balanceSlabs(misa_handle* h) {
+  Geom& G = h->s.G;
+  Walk& W = G.W;
+  const int n2 = G.n2;
+  if (n2 < 16) return MISA_OK;
+  std::vector<unsigned long long> w(static_cast<size_t>(n2));
+  HIPCHECK(h, hipMemcpy(w.data(), h->s.zwork, w.size() * 8, hipMemcpyDeviceToHost));
+  std::vector<double> P(static_cast<size_t>(n2) + 1, 0.0);
+  for (int z = 0; z < n2; z++) P[z + 1] = P[z] + static_cast<double>(w[z]);
+  if (!(P[n2] > 0.0)) return MISA_OK;
+  const int64_t row = static_cast<int64_t>(G.n1) * G.n0;
+  const int zmax = static_cast<int>(std::min<int64_t>(n2, h->bpx_cap * 256 / row));
+  int zl[9];
+  zl[0] = 0;
+  zl[8] = n2;
+  for (int c = 1; c < 8; c++) {
+    const double target = P[n2] * c / 8.0;
+    const int lo = std::max(zl[c - 1] + 1, n2 - (8 - c) * zmax);
+    const int hi = std::min(zl[c - 1] + zmax, n2 - (8 - c));
+    int z = lo;
+    while (z < hi && P[z + 1] <= target) z++;
+    if (z < hi && P[z + 1] - target < target - P[z]) z++;
+    zl[c] = z;
+  }
+  auto busiest = [&](const int* cut) {
+    double m = 0.0;
+    for (int c = 0; c < 8; c++) m = std::max(m, P[cut[c + 1]] - P[cut[c]]);
+    return m;
+  };
+  if (busiest(zl) > 0.97 * busiest(W.zlo)) return MISA_OK;
+  setSlabs(G, zl);
+  h->gen++;
+  return MISA_OK;
+}
+
 void clearEvents(misa_handle* h) {
   for (auto e : h->events) (void)hipEventDestroy(e);
   h->events.clear();
@@ -998,10 +1059,21 @@ int misa_step(misa_handle* h, int64_t nsteps) {
   HIPCHECK(h, hipEventRecord(e0, h->stream));
   enqRescanIfChanged(h);
   enqFirstAndDecide(h, p, true);
-  for (int64_t i = 0; i < nsteps; i++) enqBody(h, p, i == nsteps - 1);
+  // the first force pass of the call measures its work per plane
+  const bool measure = h->balance && p.mixed && h->s.ncu >= 8;
+  for (int64_t i = 0; i < nsteps; i++) {
+    if (i == 0 && measure) {
+      HIPCHECK(h, hipMemsetAsync(h->s.zwork, 0, static_cast<size_t>(h->s.G.n2) * 8, h->stream));
+      h->s.zmeasure = true;
+    }
+    enqBody(h, p, i == nsteps - 1);
+    h->s.zmeasure = false;
+  }
   HIPCHECK(h, hipGetLastError());
   HIPCHECK(h, hipEventRecord(e1, h->stream));
-  return finishSteps(h, nsteps, e0, e1);
+  const int rc = finishSteps(h, nsteps, e0, e1);
+  if (rc != MISA_OK || !measure || h->force_pass != MISA_FORCE_PASS_MIXED_LDS) return rc;
+  return balanceSlabs(h);
 }
 
 int misa_setv(misa_handle* h, const int32_t lat[4], const double dir[3], double energy) {
@@ -1529,6 +1601,41 @@ int misa_set_lds_tables(misa_handle* h, int32_t mode) {
     return fail(h, MISA_E_ARG, "LDS table mode must be MISA_LDS_AUTO, MISA_LDS_ON or MISA_LDS_OFF");
   h->s.lds = mode;
   h->gen++;
+  return MISA_OK;
+}
+
+int misa_set_xcd_balance(misa_handle* h, int32_t on) {
+  if (!h || (on != 0 && on != 1)) return MISA_E_ARG;
+  h->balance = on != 0;
+  if (!h->balance) {
+    evenSlabs(h->s.G);
+    h->gen++;
+  }
+  return MISA_OK;
+}
+
+int misa_set_xcd_slabs(misa_handle* h, const int32_t zlo[9]) {
+  if (!h || !zlo) return MISA_E_ARG;
+  const Geom& G = h->s.G;
+  const int64_t row = static_cast<int64_t>(G.n1) * G.n0;
+  bool ok = zlo[0] == 0 && zlo[8] == G.n2;
+  for (int c = 0; c < 8 && ok; c++)
+    ok = zlo[c + 1] >= zlo[c] && ((zlo[c + 1] - zlo[c]) * row + 255) / 256 <= h->bpx_cap;
+  if (!ok)
+    return fail(h, MISA_E_ARG,
+                "slab bounds must run from 0 to the box's planes, non-decreasing, each slab at most twice an "
+                "eighth of the box");
+  int cut[9];
+  for (int c = 0; c <= 8; c++) cut[c] = zlo[c];
+  setSlabs(h->s.G, cut);
+  h->balance = false;
+  h->gen++;
+  return MISA_OK;
+}
+
+int misa_xcd_slabs(const misa_handle* h, int32_t zlo[9]) {
+  if (!h || !zlo) return MISA_E_ARG;
+  for (int c = 0; c <= 8; c++) zlo[c] = h->s.G.W.zlo[c];
   return MISA_OK;
 }
 

@@ -685,7 +685,8 @@ __global__ __launch_bounds__(kBlockP) void kForceL(Geom G, LatOffsets O, Tables<
                                                    float4* __restrict__ recn, const float4* __restrict__ dl,
                                                    const uint64_t* __restrict__ wmask, XPtr v, XPtr f,
                                                    const uint64_t* __restrict__ id, VerletParams vp, StageArgs st,
-                                                   Ctl* ctl, InterView iv, int mlo, int nkn) {
+                                                   Ctl* ctl, InterView iv, int mlo, int nkn,
+                                                   unsigned long long* zwork) {
   __shared__ float4 offT[2 * kMaxLatOffsets];  // {lattice vector, record shift * 16}
   extern __shared__ float4 knT[];              // per knot: Fe-Fe r*phi cubic, Fe density slope cubic
   for (int i = threadIdx.x; i < 2 * kMaxLatOffsets; i += kBlockP) {
@@ -700,8 +701,10 @@ __global__ __launch_bounds__(kBlockP) void kForceL(Geom G, LatOffsets O, Tables<
   unsigned long long db = 0;  // |dx|^2 bits of the drift (MODE 1), likewise
   auto shift = [](float w) { return __float_as_uint(w); };
   forVirtualBlocks(G, [&](uint32_t b, uint32_t tl) {
+    const long long t0 = zwork ? clock64() : 0;
     forceVBlockL<ACC, MODE, U>(G, O, tb, x, xn, type, rec, recn, wmask, v, f, id, vp, st, ctl, iv, offT, knT, mlo, b,
                                tl, shift, vb, db);
+    if (zwork) addPlaneWork(G, zwork, b, tl, t0);
   });
   if (MODE == 0 && vp.vmax) waveMaxTo(vb, vp.vmax);
   if (MODE == 1) waveMaxTo(db, &ctl->dispmax2);
@@ -844,7 +847,7 @@ int launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t6
       kind = 2;                                                                                               \
       kForceL<A, M, U><<<persistentGrid(s), kBlockP, lk.bytes, st>>>(                                         \
           s.G, o, t32, x, xn, s.type, s.rec[s.cur], s.rec[1 - s.cur], s.dl, s.wmask, xp(s.v), xp(s.f), s.id, vp, \
-          sa, s.ctl, iv, lk.mlo, lk.nkn);                                                                     \
+          sa, s.ctl, iv, lk.mlo, lk.nkn, s.zmeasure ? s.zwork : nullptr);                                     \
     } else {                                                                                                  \
       kForceM<A, M, U><<<grid, kBlock, 0, st>>>(s.G, o, t32, x, xn, s.type, s.rec[s.cur], s.rec[1 - s.cur],    \
                                                 s.dl, s.wmask, xp(s.v), xp(s.f), s.id, vp, sa, s.ctl, iv);    \

@@ -85,6 +85,10 @@ struct DevState {
   // skin for some lane at the last full scan (2 words)
   double skin = 0.2;
   uint64_t* smask = nullptr;
+  // per z-plane: the LDS force pass's wavefront cycles, collected on the
+  // launches made with zmeasure set (the XCD slab balance reads them)
+  unsigned long long* zwork = nullptr;
+  bool zmeasure = false;
   InterDev in;
   Ctl* ctl = nullptr;
   double* partials = nullptr;  // reduction scratch

@@ -70,6 +70,19 @@ __device__ __forceinline__ void forVirtualBlocks(const Geom& G, Body body) {
 
 inline unsigned siteGrid(const Geom& G) { return static_cast<unsigned>(16 * G.W.bpx); }
 
+// Work per z-plane of a pass (the data the XCD slabs are balanced by): the
+// shader clock cycles a wavefront spent on its part of virtual block b, added
+// to the plane of the wavefront's first site (empty wavefronts add nothing).
+__device__ __forceinline__ void addPlaneWork(const Geom& G, unsigned long long* zw, uint32_t b, uint32_t tl,
+                                             long long t0) {
+  const long long t1 = clock64();
+  int64_t s;
+  int p, cx, cy, cz;
+  const uint32_t t = __builtin_amdgcn_readfirstlane(tl & ~63u);
+  if (siteOfBlock(G, b, t, s, p, cx, cy, cz) && (threadIdx.x & 63u) == 0)
+    atomicAdd(zw + cz, static_cast<unsigned long long>(t1 - t0));
+}
+
 __device__ __forceinline__ bool nearFace(const Geom& G, int cx, int cy, int cz) {
   return cx < G.g || cx >= G.n0 - G.g || cy < G.g || cy >= G.n1 - G.g || cz < G.g || cz >= G.n2 - G.g;
 }

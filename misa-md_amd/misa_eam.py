@@ -141,6 +141,9 @@ _SIGS = {
     "misa_set_launch_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "misa_set_lds_tables": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "misa_force_pass_used": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "misa_set_xcd_balance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "misa_xcd_slabs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "misa_set_xcd_slabs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "misa_defect_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "misa_get_launch_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
                                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
@@ -354,6 +357,21 @@ class EamDevice:
         k = ctypes.c_int32()
         self._check(self.lib.misa_force_pass_used(self.h, ctypes.byref(k)), "misa_force_pass_used")
         return {0: "kForce", 1: "kForceM", 2: "kForceL"}.get(k.value, "none")
+
+    def set_xcd_balance(self, on):
+        """Cut the XCD z-slabs by the force pass's measured work (default on)."""
+        self._check(self.lib.misa_set_xcd_balance(self.h, 1 if on else 0), "misa_set_xcd_balance")
+
+    def xcd_slabs(self):
+        """Slab bounds: XCD c walks z-planes [zlo[c], zlo[c+1])."""
+        z = (ctypes.c_int32 * 9)()
+        self._check(self.lib.misa_xcd_slabs(self.h, z), "misa_xcd_slabs")
+        return list(z)
+
+    def set_xcd_slabs(self, zlo):
+        """Set the slab bounds (9 values from 0 to the box's planes) and turn balancing off."""
+        z = (ctypes.c_int32 * 9)(*zlo)
+        self._check(self.lib.misa_set_xcd_slabs(self.h, z), "misa_set_xcd_slabs")
 
     def launch_shape(self):
         a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
