@@ -141,6 +141,8 @@ int buildPlaneOffsets(misa_handle* h) {
     h->lat.n[p] = nl;
     h->fullPlane.n[p] = static_cast<int>(L.size());
     h->fullPlane.nw[p] = nw;
+    for (int k = static_cast<int>(L.size()); k < kMaxFullOffsets; k++) h->fullPlane.off[p][k] = 0;
+    for (int k = nw; k < kMaxFullOffsets; k++) h->fullPlane.woff[p][k] = 0;
   }
   if (h->fullPlane.n[0] != h->fullPlane.n[1]) return fail(h, MISA_E_ARG, "even and odd offset lists differ in length");
   if (h->s.in.full) {

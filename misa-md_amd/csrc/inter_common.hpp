@@ -14,12 +14,16 @@ namespace misa {
 // The reference's full neighbour lists (NeighbourIndex even/odd,
 // neighbour_index.inl:13-76) as parity-plane slot deltas; inter atoms and the
 // lattice sites around them walk all of them.
+// The site kernels read the offsets eight at a time (one scalar load of 32
+// bytes): the arrays come first, 32-byte aligned, and the entries from n
+// (nw) on are 0, the site itself.
+static_assert(kMaxFullOffsets % 8 == 0, "offset rows are read eight at a time");
 struct FullPlaneOffsets {
-  int n[2];
-  int32_t off[2][kMaxFullOffsets];
+  alignas(32) int32_t off[2][kMaxFullOffsets];
   // the full-list offsets missing from the pruned lattice list (kWide sites)
+  alignas(32) int32_t woff[2][kMaxFullOffsets];
+  int n[2];
   int nw[2];
-  int32_t woff[2][kMaxFullOffsets];
 };
 
 // What the site kernels need to see the inter atoms.
@@ -139,34 +143,59 @@ __device__ __forceinline__ void forBucket(const InterView& iv, int64_t slot, int
   }
 }
 
+// Wave-uniform reads of data no kernel writes while it runs (the offset
+// tables), through the scalar cache: a read through a plain pointer in a
+// kernel that also stores to global memory goes through the vector memory
+// path, and each such read then holds up the gather that needs it.
+template <typename T>
+__device__ __forceinline__ T scalarRead(const T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  using CT = const __attribute__((address_space(4))) T;
+  return *(CT*)(p);
+#else
+  return *p;  // the host pass only parses device code
+#endif
+}
+struct alignas(32) Offsets8 {
+  int32_t o[8];
+};
+// Entries q0 .. q0 + 7 of an offset row (q0 a multiple of 8).
+__device__ __forceinline__ Offsets8 offsets8(const int32_t* row, int q0) {
+  return scalarRead(reinterpret_cast<const Offsets8*>(row + q0));
+}
+
+// Visits the entries of bucket `slot` in insertion order, from its head h.
+template <typename Fn>
+__device__ __forceinline__ void followBucket(const InterView& iv, int64_t slot, int h, int ne, Fn& fn) {
+  for (int e = h; e < ne; ++e) {
+    const uint64_t k = iv.keys[e];
+    if (static_cast<int64_t>(k >> kEntryBits) != slot) break;
+    fn(static_cast<int>(k & ((1u << kEntryBits) - 1)));
+  }
+}
+
 // Visits, in full-list order (the site itself first), the entries of the
-// buckets of site s (parity p) and of its full-list neighbours.  The bucket
-// heads are read eight at a time before any bucket is followed: nearly all
-// are empty, and the reads then overlap instead of queueing one round trip
-// behind the other.
+// buckets of site s (parity p, the same over the wavefront) and of its
+// full-list neighbours.  The bucket heads are read eight at a time before
+// any bucket is followed: nearly all are empty, and the reads then overlap
+// instead of queueing one round trip behind the other.
 template <typename Fn>
 __device__ __forceinline__ void forNearBuckets(const InterView& iv, const FullPlaneOffsets& full, int p, int64_t s,
                                                int ne, Fn fn) {
-  constexpr int B = 4;
-  const int n = full.n[p];
-  for (int q0 = -1; q0 < n; q0 += B) {
-    int32_t h[B];
+  const int n = scalarRead(&full.n[p]);
+  const int32_t* off = full.off[p];
+  {
+    const int h0 = iv.head[s];
+    if (h0 >= 0) followBucket(iv, s, h0, ne, fn);
+  }
+  for (int q0 = 0; q0 < n; q0 += 8) {
+    const Offsets8 o = offsets8(off, q0);
+    int32_t h[8];
 #pragma unroll
-    for (int u = 0; u < B; ++u) {
-      const int q = q0 + u;
-      h[u] = q < n ? iv.head[q < 0 ? s : s + full.off[p][q]] : -1;
-    }
+    for (int u = 0; u < 8; ++u) h[u] = iv.head[s + o.o[u]];
 #pragma unroll
-    for (int u = 0; u < B; ++u) {
-      if (h[u] < 0) continue;
-      const int q = q0 + u;
-      const int64_t slot = q < 0 ? s : s + full.off[p][q];
-      for (int e = h[u]; e < ne; ++e) {
-        const uint64_t k = iv.keys[e];
-        if (static_cast<int64_t>(k >> kEntryBits) != slot) break;
-        fn(static_cast<int>(k & ((1u << kEntryBits) - 1)));
-      }
-    }
+    for (int u = 0; u < 8; ++u)
+      if (q0 + u < n && h[u] >= 0) followBucket(iv, s + o.o[u], h[u], ne, fn);
   }
 }
 
@@ -177,18 +206,16 @@ __device__ __forceinline__ void forNearBuckets(const InterView& iv, const FullPl
 template <typename Fn>
 __device__ __forceinline__ void forWideTargets(const FullPlaneOffsets& full, const int8_t* __restrict__ type, int p,
                                                int64_t s, int8_t ti, Fn fn) {
-  constexpr int B = 4;
-  const int n = full.nw[p];
-  for (int q0 = 0; q0 < n; q0 += B) {
-    int8_t t[B];
+  const int n = scalarRead(&full.nw[p]);
+  const int32_t* woff = full.woff[p];
+  for (int q0 = 0; q0 < n; q0 += 8) {
+    const Offsets8 o = offsets8(woff, q0);
+    int8_t t[8];
 #pragma unroll
-    for (int u = 0; u < B; ++u) {
-      const int q = q0 + u;
-      t[u] = q < n ? type[s + full.woff[p][q]] : kVacant;
-    }
+    for (int u = 0; u < 8; ++u) t[u] = type[s + o.o[u]];
 #pragma unroll
-    for (int u = 0; u < B; ++u)
-      if (wideTarget(ti, t[u])) fn(s + full.woff[p][q0 + u], t[u]);
+    for (int u = 0; u < 8; ++u)
+      if (q0 + u < n && wideTarget(ti, t[u])) fn(s + o.o[u], t[u]);
   }
 }
 
