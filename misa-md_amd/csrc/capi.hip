@@ -149,10 +149,6 @@ int buildPlaneOffsets(misa_handle* h) {
     hipError_t e = hipMemcpy(h->s.in.full, &h->fullPlane, sizeof(FullPlaneOffsets), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail(h, MISA_E_HIP, std::string("hipMemcpy offsets: ") + hipGetErrorString(e));
   }
-  if (h->s.dl) {
-    hipError_t e = hipMemcpy(h->s.dl, h->dlHost, sizeof(h->dlHost), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return fail(h, MISA_E_HIP, std::string("hipMemcpy offsets: ") + hipGetErrorString(e));
-  }
   // non-negative form for the buffer loads: sh = off - lo, lo = most negative delta
   int32_t lo = 0;
   for (int p = 0; p < 2; p++)
@@ -160,6 +156,21 @@ int buildPlaneOffsets(misa_handle* h) {
   h->lat.lo = lo;
   for (int p = 0; p < 2; p++)
     for (int k = 0; k < h->lat.n[p]; k++) h->lat.sh[p][k] = static_cast<uint32_t>(h->lat.off[p][k] - lo);
+  if (h->s.dl) {
+    // device table: the lattice vectors with the packed deltas in w, then
+    // the same vectors with the record shifts * 16 in w
+    std::vector<float4> dev(4 * kMaxLatOffsets);
+    for (int i = 0; i < 2 * kMaxLatOffsets; i++) {
+      dev[i] = h->dlHost[i];
+      const int p = i / kMaxLatOffsets, k = i % kMaxLatOffsets;
+      const uint32_t sh16 = k < h->lat.n[p] ? h->lat.sh[p][k] * 16u : 0u;
+      float w;
+      std::memcpy(&w, &sh16, sizeof(w));
+      dev[2 * kMaxLatOffsets + i] = make_float4(h->dlHost[i].x, h->dlHost[i].y, h->dlHost[i].z, w);
+    }
+    hipError_t e = hipMemcpy(h->s.dl, dev.data(), dev.size() * sizeof(float4), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(h, MISA_E_HIP, std::string("hipMemcpy offsets: ") + hipGetErrorString(e));
+  }
   h->have_offsets = true;
   h->gen++;
   return MISA_OK;
@@ -393,7 +404,7 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
   if (rc == MISA_OK) rc |= devAlloc(h, &s.df64, slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.rec[0], slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.rec[1], slots);
-  if (rc == MISA_OK) rc |= devAlloc(h, &s.dl, 2 * kMaxLatOffsets);
+  if (rc == MISA_OK) rc |= devAlloc(h, &s.dl, 4 * kMaxLatOffsets);
   // a balanced slab may hold up to twice the planes of an even cut
   h->bpx_cap = 2 * G.W.bpx;
   if (rc == MISA_OK) rc |= devAlloc(h, &s.wmask, static_cast<size_t>(2 * 4 * 16 * h->bpx_cap));

@@ -468,63 +468,58 @@ __device__ __forceinline__ void rhoPassM(const Geom& G, const LatOffsets& O, con
     const Rsrc rr = rsrcBelow(rec, O.lo);
     const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoH));
     const uint32_t o16 = static_cast<uint32_t>(s) * 16u;
-    const uint32_t* sh = O.sh[p];
-    const float4* D = dl + p * kMaxLatOffsets;
+    // per pruned offset: the lattice vector and, in w, the record shift * 16
+    const float4* D = dl + (2 + p) * kMaxLatOffsets;
     const float rc2 = static_cast<float>(G.rc2);
     const int nk = O.n[p];
-    auto group = [&](auto UC, const int* ks) {
-      constexpr int V = decltype(UC)::value;
-      float4 nb[V], d[V];
-      bool in[V], sk[V];
+    // U offsets at a time; the entries past the last repeat the group's first
+    // offset with a lattice vector out of range, so they add nothing
+    auto group = [&](const int (&ks)[U], const bool (&act)[U]) {
+      float4 nb[U], d[U];
+      bool in[U], sk[U];
 #pragma unroll
-      for (int u = 0; u < V; ++u) {
-        nb[u] = ldRec4(rr, o16, sh[ks[u]] * 16u);
+      for (int u = 0; u < U; ++u) {
         d[u] = D[ks[u]];
+        if (!act[u]) d[u].x = d[u].y = d[u].z = 1e18f;
       }
-      rhoGroupM<ACC, V>(me, nb, d, rc2, rcs2, rtab, tb, acc, in, sk);
 #pragma unroll
-      for (int u = 0; u < V; ++u) {
+      for (int u = 0; u < U; ++u) nb[u] = ldRec4(rr, o16, __float_as_uint(d[u].w));
+      rhoGroupM<ACC, U>(me, nb, d, rc2, rcs2, rtab, tb, acc, in, sk);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
         if (__ballot(in[u])) setBit(w0, w1, ks[u]);
         if (!skin && __ballot(sk[u])) setBit(s0, s1, ks[u]);
       }
     };
+    int ks[U];
+    bool act[U];
     if (!skin) {
-      int ks[U];
-      int k = 0;
-      for (; k + U <= nk; k += U) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) ks[u] = k + u;
-        group(std::integral_constant<int, U>{}, ks);
-      }
-      for (; k < nk; ++k) {
-        ks[0] = k;
-        group(std::integral_constant<int, 1>{}, ks);
-      }
-    } else {
-      // (the loop of forMaskGroups written out: through the helper this
-      // kernel keeps the offsets on the stack)
-      uint64_t b0 = uniform64(smask[2 * wv]), b1 = uniform64(smask[2 * wv + 1]);
-      int left = __builtin_popcountll(b0) + __builtin_popcountll(b1);
-      for (; left > 0;) {
-        const int V = left >= U ? U : 1;
-        int ks[U];
+      for (int k = 0; k < nk; k += U) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          if (u < V) {
-            if (b0) {
-              ks[u] = __builtin_ctzll(b0);
-              b0 &= b0 - 1;
-            } else {
-              ks[u] = 64 + __builtin_ctzll(b1);
-              b1 &= b1 - 1;
-            }
-          } else {
-            ks[u] = 0;
-          }
+          act[u] = k + u < nk;
+          ks[u] = act[u] ? k + u : k;
         }
-        if (V == U) group(std::integral_constant<int, U>{}, ks);
-        else group(std::integral_constant<int, 1>{}, ks);
-        left -= V;
+        group(ks, act);
+      }
+    } else {
+      // the offsets of the last full scan's skin mask, in ascending order
+      uint64_t b0 = uniform64(smask[2 * wv]), b1 = uniform64(smask[2 * wv + 1]);
+      while (b0 | b1) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          act[u] = (b0 | b1) != 0;
+          int k;
+          if (b0) {
+            k = __builtin_ctzll(b0);
+            b0 &= b0 - 1;
+          } else {
+            k = b1 ? 64 + __builtin_ctzll(b1) : 0;
+            b1 &= b1 - 1;
+          }
+          ks[u] = act[u] ? k : ks[0];
+        }
+        group(ks, act);
       }
     }
   }

@@ -35,50 +35,54 @@ namespace {
 
 using namespace site;
 
-// firststep from stored forces + lattice decide + position halo.
+// firststep from stored forces + lattice decide + position halo, in a grid
+// that strides over the virtual blocks of a site launch (a whole number of
+// blocks per XCD keeps each on its XCD's slab), so that few wavefronts add
+// their largest drift to the one word every wavefront of the pass feeds.
 __global__ __launch_bounds__(kBlock) void kFirst(Geom G, CXPtr x, XPtr xn, const int8_t* __restrict__ type,
                                                  XPtr v, CXPtr f, const uint64_t* __restrict__ id, VerletParams vp,
-                                                 StageArgs st, Ctl* ctl, float4* __restrict__ recn) {
+                                                 StageArgs st, Ctl* ctl, float4* __restrict__ recn, uint32_t nb) {
 #pragma clang fp contract(off)  // the half kick and drift round as newton_motion.cpp does
-  unsigned long long db = 0;  // |dx|^2 bits of the drift
-  [&] {
-  int64_t s;
-  int p, cx, cy, cz;
-  if (!siteOfThread(G, s, p, cx, cy, cz)) return;
-  const int8_t ti = type[s];
-  double y0 = x.c[0][s], y1 = x.c[1][s], y2 = x.c[2][s];
-  if (!isVacant(ti)) {
-    const StepDt sd = stepDt(vp, species(ti));
-    const double c = sd.c;
-    double v0 = v.c[0][s] + c * f.c[0][s];
-    double v1 = v.c[1][s] + c * f.c[1][s];
-    double v2 = v.c[2][s] + c * f.c[2][s];
-    y0 = y0 + sd.dt * v0;
-    y1 = y1 + sd.dt * v1;
-    y2 = y2 + sd.dt * v2;
-    db = speed2Bits(y0 - x.c[0][s], y1 - x.c[1][s], y2 - x.c[2][s]);
-    if (leftSite(G, p, cx, cy, cz, y0, y1, y2)) {
-      stageInter(st, ctl, refIndex(G, p, cx, cy, cz), s, id[s], ti, y0, y1, y2, v0, v1, v2);
-      v0 = v1 = v2 = 0.0;
+  unsigned long long db = 0;  // |dx|^2 bits of the drift, largest over the lane's sites
+  for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    int64_t s;
+    int p, cx, cy, cz;
+    if (!siteOfBlock(G, b, threadIdx.x, s, p, cx, cy, cz)) continue;
+    const int8_t ti = type[s];
+    double y0 = x.c[0][s], y1 = x.c[1][s], y2 = x.c[2][s];
+    if (!isVacant(ti)) {
+      const StepDt sd = stepDt(vp, species(ti));
+      const double c = sd.c;
+      double v0 = v.c[0][s] + c * f.c[0][s];
+      double v1 = v.c[1][s] + c * f.c[1][s];
+      double v2 = v.c[2][s] + c * f.c[2][s];
+      y0 = y0 + sd.dt * v0;
+      y1 = y1 + sd.dt * v1;
+      y2 = y2 + sd.dt * v2;
+      const unsigned long long d = speed2Bits(y0 - x.c[0][s], y1 - x.c[1][s], y2 - x.c[2][s]);
+      db = d > db ? d : db;
+      if (leftSite(G, p, cx, cy, cz, y0, y1, y2)) {
+        stageInter(st, ctl, refIndex(G, p, cx, cy, cz), s, id[s], ti, y0, y1, y2, v0, v1, v2);
+        v0 = v1 = v2 = 0.0;
+      }
+      v.c[0][s] = v0;
+      v.c[1][s] = v1;
+      v.c[2][s] = v2;
     }
-    v.c[0][s] = v0;
-    v.c[1][s] = v1;
-    v.c[2][s] = v2;
+    xn.c[0][s] = y0;
+    xn.c[1][s] = y1;
+    xn.c[2][s] = y2;
+    forEachImage(G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
+      xn.c[0][q] = y0 + sx * G.L0;
+      xn.c[1][q] = y1 + sy * G.L1;
+      xn.c[2][q] = y2 + sz * G.L2;
+    });
+    if (recn) {
+      const float4 r = isVacant(ti) ? vacantRec() : siteRec(G, p, cx, cy, cz, y0, y1, y2, species(ti));
+      recn[s] = r;
+      forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) { recn[q] = r; });
+    }
   }
-  xn.c[0][s] = y0;
-  xn.c[1][s] = y1;
-  xn.c[2][s] = y2;
-  forEachImage(G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
-    xn.c[0][q] = y0 + sx * G.L0;
-    xn.c[1][q] = y1 + sy * G.L1;
-    xn.c[2][q] = y2 + sz * G.L2;
-  });
-  if (recn) {
-    const float4 r = isVacant(ti) ? vacantRec() : siteRec(G, p, cx, cy, cz, y0, y1, y2, species(ti));
-    recn[s] = r;
-    forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) { recn[q] = r; });
-  }
-  }();
   waveMaxTo(db, &ctl->dispmax2);
 }
 
@@ -312,8 +316,10 @@ void launchPackRec(const DevState& s, int buf, hipStream_t st) {
 }
 
 void launchFirst(const DevState& s, bool mixed, const VerletParams& vp, hipStream_t st) {
-  kFirst<<<siteGrid(s.G), kBlock, 0, st>>>(s.G, cxp(s.x[s.cur]), xp(s.x[1 - s.cur]), s.type, xp(s.v), cxp(s.f),
-                                           s.id, vp, stageOf(s), s.ctl, mixed ? s.rec[1 - s.cur] : nullptr);
+  const unsigned nb = siteGrid(s.G);
+  kFirst<<<strideGrid(s.G, s.ncu), kBlock, 0, st>>>(s.G, cxp(s.x[s.cur]), xp(s.x[1 - s.cur]), s.type, xp(s.v),
+                                                    cxp(s.f), s.id, vp, stageOf(s), s.ctl,
+                                                    mixed ? s.rec[1 - s.cur] : nullptr, nb);
 }
 
 int64_t launchThermo(const DevState& s, hipStream_t st) {

@@ -68,7 +68,9 @@ struct DevState {
   int8_t* type = nullptr;
   uint64_t* id = nullptr;
   // mixed precision: site records {u, F'} per slot (ping-pong with x) and the
-  // lattice vector of every pruned offset per centre parity
+  // lattice vector of every pruned offset per centre parity, twice: with the
+  // packed plane delta in w (2 x kMaxLatOffsets), then with the record
+  // shift * 16 in w (2 x kMaxLatOffsets)
   float4* rec[2] = {nullptr, nullptr};
   float4* dl = nullptr;
   // per wavefront of the site kernels: the pruned offsets within the cutoff

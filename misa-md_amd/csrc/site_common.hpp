@@ -69,6 +69,12 @@ __device__ __forceinline__ void forVirtualBlocks(const Geom& G, Body body) {
 }
 
 inline unsigned siteGrid(const Geom& G) { return static_cast<unsigned>(16 * G.W.bpx); }
+// Grid of a pass that strides over the siteGrid virtual blocks: up to 8
+// blocks per compute unit, a whole number per XCD.
+inline unsigned strideGrid(const Geom& G, int ncu) {
+  const unsigned g = static_cast<unsigned>(ncu >= 8 ? ncu / 8 * 8 * 8 : 8);
+  return siteGrid(G) < g ? siteGrid(G) : g;
+}
 
 // Work per z-plane of a pass (the data the XCD slabs are balanced by): the
 // shader clock cycles a wavefront spent on its part of virtual block b, added
