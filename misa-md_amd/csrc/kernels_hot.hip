@@ -95,6 +95,33 @@ __device__ __forceinline__ void forMaskGroups(uint64_t b0, uint64_t b1, Visit vi
   }
 }
 
+// Calls visit(ks, act) for the set bits of (b0, b1) in increasing order,
+// always U at a time: in the last group the entries past the last bit
+// (act[u] false) repeat the last offset, and the caller makes them add
+// nothing.  One shape of group keeps the gathers of a group in flight
+// together.
+template <int U, typename Visit>
+__device__ __forceinline__ void forMaskGroupsFull(uint64_t b0, uint64_t b1, Visit visit) {
+  int k = 0;
+  while (b0 | b1) {
+    int ks[U];
+    bool act[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      act[u] = (b0 | b1) != 0;
+      if (b0) {
+        k = __builtin_ctzll(b0);
+        b0 &= b0 - 1;
+      } else if (b1) {
+        k = 64 + __builtin_ctzll(b1);
+        b1 &= b1 - 1;
+      }
+      ks[u] = k;
+    }
+    visit(ks, act);
+  }
+}
+
 // Set bit k of the 128-bit offset mask (m0, m1).
 __device__ __forceinline__ void setBit(uint64_t& m0, uint64_t& m1, int k) {
   if (k < 64) m0 |= 1ull << k;
@@ -581,19 +608,17 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
   Sum3<ACC> acc;
   // the offsets the density pass found in range for some lane of this wavefront
   const uint32_t wv = waveId();
-  auto group = [&](auto UC, const int* ks) {
-    constexpr int V = decltype(UC)::value;
-    float4 nb[V], d[V];
+  // a group's entries past the mask's last offset get a lattice vector out
+  // of range: beyond the cutoff, they add nothing
+  forMaskGroupsFull<U>(uniform64(wmask[2 * wv]), uniform64(wmask[2 * wv + 1]), [&](const int* ks, const bool* act) {
+    float4 nb[U], d[U];
 #pragma unroll
-    for (int u = 0; u < V; ++u) {
+    for (int u = 0; u < U; ++u) {
       nb[u] = ldRec4(rr, o16, sh[ks[u]] * 16u);
       d[u] = D[ks[u]];
+      if (!act[u]) d[u].x = d[u].y = d[u].z = 1e18f;
     }
-    forceGroupM<ACC, V>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, ks, acc);
-  };
-  forMaskGroups<U>(uniform64(wmask[2 * wv]), uniform64(wmask[2 * wv + 1]), [&](const int* ks, int nv) {
-    if (nv == U) group(std::integral_constant<int, U>{}, ks);
-    else group(std::integral_constant<int, 1>{}, ks);
+    forceGroupM<ACC, U>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, ks, acc);
   });
   flushChunk3(acc);
   forceTailM<ACC, MODE>(G, s, p, cx, cy, cz, ti, ei, dfi, acc, x, xn, type, rec, recn, v, f, id, vp, st, ctl, tb,
@@ -636,19 +661,17 @@ __device__ __forceinline__ void forceVBlockL(const Geom& G, const LatOffsets& O,
   const float4* T = offT + p * kMaxLatOffsets;
   Sum3<ACC> acc;
   const uint32_t wv = b * (kBlock / 64) + __builtin_amdgcn_readfirstlane(tl >> 6);
-  auto group = [&](auto UC, const int* ks) {
-    constexpr int V = decltype(UC)::value;
-    float4 nb[V], d[V];
+  // a group's entries past the mask's last offset get a lattice vector out
+  // of range: beyond the cutoff, they add nothing
+  forMaskGroupsFull<U>(uniform64(wmask[2 * wv]), uniform64(wmask[2 * wv + 1]), [&](const int* ks, const bool* act) {
+    float4 nb[U], d[U];
 #pragma unroll
-    for (int u = 0; u < V; ++u) {
+    for (int u = 0; u < U; ++u) {
       d[u] = T[ks[u]];
       nb[u] = ldRec4(rr, o16 + shift(d[u].w), 0);
+      if (!act[u]) d[u].x = d[u].y = d[u].z = 1e18f;
     }
-    forceGroupL<ACC, V>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, knT, mlo, ks, acc);
-  };
-  forMaskGroups<U>(uniform64(wmask[2 * wv]), uniform64(wmask[2 * wv + 1]), [&](const int* ks, int nv) {
-    if (nv == U) group(std::integral_constant<int, U>{}, ks);
-    else group(std::integral_constant<int, 1>{}, ks);
+    forceGroupL<ACC, U>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, knT, mlo, ks, acc);
   });
   flushChunk3(acc);
   unsigned long long vbs = 0, dbs = 0;
