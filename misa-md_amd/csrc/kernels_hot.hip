@@ -666,8 +666,9 @@ __device__ __forceinline__ void forceVBlockL(const Geom& G, const LatOffsets& O,
   forMaskGroupsFull<U>(uniform64(wmask[2 * wv]), uniform64(wmask[2 * wv + 1]), [&](const int* ks, const bool* act) {
     float4 nb[U], d[U];
 #pragma unroll
+    for (int u = 0; u < U; ++u) d[u] = T[ks[u]];
+#pragma unroll
     for (int u = 0; u < U; ++u) {
-      d[u] = T[ks[u]];
       nb[u] = ldRec4(rr, o16 + shift(d[u].w), 0);
       if (!act[u]) d[u].x = d[u].y = d[u].z = 1e18f;
     }
