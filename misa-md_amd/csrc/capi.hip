@@ -123,7 +123,7 @@ int buildPlaneOffsets(misa_handle* h) {
         h->fullPlane.woff[p][nw++] =
             static_cast<int32_t>(po.dp * NX * NY * NZ + (static_cast<int64_t>(po.dz) * NY + po.dy) * NX + po.dcx);
       } else {
-        if (nl >= kMaxLatOffsets) return fail(h, MISA_E_ARG, "pruned offset list too long");
+        if (nl >= kPadOffset) return fail(h, MISA_E_ARG, "pruned offset list too long");
         const int64_t NX = b.n[0] + 2 * b.g, NY = b.n[1] + 2 * b.g, NZ = b.n[2] + 2 * b.g;
         const int64_t plane = NX * NY * NZ;
         const int64_t d = po.dp * plane + (static_cast<int64_t>(po.dz) * NY + po.dy) * NX + po.dcx;
@@ -139,6 +139,8 @@ int buildPlaneOffsets(misa_handle* h) {
       }
     }
     h->lat.n[p] = nl;
+    for (int k = nl; k < kMaxLatOffsets; k++)  // the padding entries (kPadOffset among them)
+      h->dlHost[p * kMaxLatOffsets + k] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
     h->fullPlane.n[p] = static_cast<int>(L.size());
     h->fullPlane.nw[p] = nw;
     for (int k = static_cast<int>(L.size()); k < kMaxFullOffsets; k++) h->fullPlane.off[p][k] = 0;
@@ -154,8 +156,30 @@ int buildPlaneOffsets(misa_handle* h) {
   for (int p = 0; p < 2; p++)
     for (int k = 0; k < h->lat.n[p]; k++) lo = std::min(lo, h->lat.off[p][k]);
   h->lat.lo = lo;
-  for (int p = 0; p < 2; p++)
+  for (int p = 0; p < 2; p++) {
     for (int k = 0; k < h->lat.n[p]; k++) h->lat.sh[p][k] = static_cast<uint32_t>(h->lat.off[p][k] - lo);
+    // the padding entries read the site's own record (delta 0): a valid
+    // address for either parity, whereas lo alone lies below the array for
+    // the even sites
+    for (int k = h->lat.n[p]; k < kMaxLatOffsets; k++) h->lat.sh[p][k] = static_cast<uint32_t>(-lo);
+  }
+  {
+    // every entry, the padding included, must address a slot of the array
+    // from every sub-box site of its parity (slots grow with cx, cy, cz, so
+    // the first and the last site bound them)
+    const int64_t NX = b.n[0] + 2 * b.g, NY = b.n[1] + 2 * b.g, NZ = b.n[2] + 2 * b.g;
+    const int64_t plane = NX * NY * NZ;
+    for (int p = 0; p < 2; p++) {
+      const int64_t first = p * plane + (static_cast<int64_t>(b.g) * NY + b.g) * NX + b.g;
+      const int64_t last = p * plane + (static_cast<int64_t>(b.n[2] - 1 + b.g) * NY + (b.n[1] - 1 + b.g)) * NX +
+                           (b.n[0] - 1 + b.g);
+      for (int k = 0; k < kMaxLatOffsets; k++) {
+        const int64_t delta = static_cast<int64_t>(h->lat.sh[p][k]) + lo;
+        if (first + delta < 0 || last + delta >= 2 * plane)
+          return fail(h, MISA_E_ARG, "a pruned offset reaches beyond the site arrays");
+      }
+    }
+  }
   if (h->s.dl) {
     // device table: the lattice vectors with the packed deltas in w, then
     // the same vectors with the record shifts * 16 in w
@@ -163,7 +187,7 @@ int buildPlaneOffsets(misa_handle* h) {
     for (int i = 0; i < 2 * kMaxLatOffsets; i++) {
       dev[i] = h->dlHost[i];
       const int p = i / kMaxLatOffsets, k = i % kMaxLatOffsets;
-      const uint32_t sh16 = k < h->lat.n[p] ? h->lat.sh[p][k] * 16u : 0u;
+      const uint32_t sh16 = h->lat.sh[p][k] * 16u;
       float w;
       std::memcpy(&w, &sh16, sizeof(w));
       dev[2 * kMaxLatOffsets + i] = make_float4(h->dlHost[i].x, h->dlHost[i].y, h->dlHost[i].z, w);

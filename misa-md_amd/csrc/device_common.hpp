@@ -28,6 +28,11 @@
 namespace misa {
 
 constexpr int kMaxLatOffsets = 128;   // pruned lattice-lattice list per parity
+// Entry of the pruned-offset tables past every list (at most kPadOffset
+// offsets): a lattice vector of 1e18 A and a zero record shift.  The mixed
+// passes visit offsets in full groups and point a group's entries past the
+// list there, so those pairs are beyond the cutoff and add nothing.
+constexpr int kPadOffset = kMaxLatOffsets - 1;
 constexpr int kMaxFullOffsets = 256;  // reference full list per parity
 constexpr int kKnot = 8;              // values per knot record: c0 c1 c2 _ c3 c4 c5 c6
 constexpr int8_t kVacant = -1;
@@ -90,8 +95,11 @@ struct Ctl {
   int64_t staged_total;
   int64_t refill_total;
   int64_t far_total;
+  // staged atoms put back on their own site ahead of the decide's list work
+  // in this step (kQuickRefill), whose stage entries the decide skips
+  int64_t n_quick;
 };
-static_assert(sizeof(Ctl) == 16 * 8, "Ctl layout");
+static_assert(sizeof(Ctl) == 17 * 8, "Ctl layout");
 
 // Bits of Ctl::err_flags, set by the kernels and read after every call.
 constexpr int64_t kErrInterCap = 1;   // inter-atom list capacity exceeded

@@ -500,15 +500,12 @@ __device__ __forceinline__ void rhoPassM(const Geom& G, const LatOffsets& O, con
     const float rc2 = static_cast<float>(G.rc2);
     const int nk = O.n[p];
     // U offsets at a time; the entries past the last repeat the group's first
-    // offset with a lattice vector out of range, so they add nothing
+    // offset, with the padding entry's lattice vector: they add nothing
     auto group = [&](const int (&ks)[U], const bool (&act)[U]) {
       float4 nb[U], d[U];
       bool in[U], sk[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        d[u] = D[ks[u]];
-        if (!act[u]) d[u].x = d[u].y = d[u].z = 1e18f;
-      }
+      for (int u = 0; u < U; ++u) d[u] = D[act[u] ? ks[u] : kPadOffset];
 #pragma unroll
       for (int u = 0; u < U; ++u) nb[u] = ldRec4(rr, o16, __float_as_uint(d[u].w));
       rhoGroupM<ACC, U>(me, nb, d, rc2, rcs2, rtab, tb, acc, in, sk);
@@ -614,9 +611,9 @@ __global__ __launch_bounds__(kBlock) void kForceM(Geom G, LatOffsets O, Tables<f
     float4 nb[U], d[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      nb[u] = ldRec4(rr, o16, sh[ks[u]] * 16u);
-      d[u] = D[ks[u]];
-      if (!act[u]) d[u].x = d[u].y = d[u].z = 1e18f;
+      const int k = act[u] ? ks[u] : kPadOffset;
+      nb[u] = ldRec4(rr, o16, sh[k] * 16u);
+      d[u] = D[k];
     }
     forceGroupM<ACC, U>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, ks, acc);
   });
@@ -666,12 +663,9 @@ __device__ __forceinline__ void forceVBlockL(const Geom& G, const LatOffsets& O,
   forMaskGroupsFull<U>(uniform64(wmask[2 * wv]), uniform64(wmask[2 * wv + 1]), [&](const int* ks, const bool* act) {
     float4 nb[U], d[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) d[u] = T[ks[u]];
+    for (int u = 0; u < U; ++u) d[u] = T[act[u] ? ks[u] : kPadOffset];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      nb[u] = ldRec4(rr, o16 + shift(d[u].w), 0);
-      if (!act[u]) d[u].x = d[u].y = d[u].z = 1e18f;
-    }
+    for (int u = 0; u < U; ++u) nb[u] = ldRec4(rr, o16 + shift(d[u].w), 0);
     forceGroupL<ACC, U>(me, dfi, ei, pairBase, nb, d, rc2, rtab, rtabD, tb, knT, mlo, ks, acc);
   });
   flushChunk3(acc);
@@ -711,7 +705,7 @@ __global__ __launch_bounds__(kBlockP) void kForceL(Geom G, LatOffsets O, Tables<
   for (int i = threadIdx.x; i < 2 * kMaxLatOffsets; i += kBlockP) {
     const int q = i / kMaxLatOffsets, k = i - q * kMaxLatOffsets;
     float4 d = dl[i];
-    d.w = __uint_as_float(k < O.n[q] ? O.sh[q][k] * 16u : 0u);
+    d.w = __uint_as_float(O.sh[q][k] * 16u);
     offT[i] = d;
   }
   fillKnots(tb, knT, mlo, nkn);

@@ -295,18 +295,21 @@ __device__ __forceinline__ void decideBody(const DecideArgs& A) {
   if (A.mode == 0) {
     // P1: append the new inter atoms in the order of the sites they left
     // (atom.cpp:28-53 walks the sub-box z -> y -> x and appends): sort the
-    // staged atoms by the reference linear index of their site
-    int ns = static_cast<int>(A.ctl->n_stage);
-    if (ns > in.cap) ns = static_cast<int>(in.cap);
+    // staged atoms by the reference linear index of their site.  Entries
+    // that kQuickRefill already put back carry kQuickDone and sort last.
+    const int nsAll = static_cast<int>(min(A.ctl->n_stage, in.cap));
+    const int nq = static_cast<int>(A.ctl->n_quick);
+    int ns = nsAll - nq;
     if (nl + ns > in.cap) {
       if (tid == 0) setFlag(err, 1ull);
       ns = static_cast<int>(in.cap) - nl;
     }
-    staged = ns;
-    for (int i = tid; i < ns; i += kDT)
+    staged = nsAll;
+    refills = nq;
+    for (int i = tid; i < nsAll; i += kDT)
       in.keys[i] = (static_cast<uint64_t>(in.stage_key[i]) << 21) | static_cast<uint64_t>(i);
     __syncthreads();
-    sortKeys(in.keys, ns, sKeys);
+    sortKeys(in.keys, nsAll, sKeys);
     for (int r = tid; r < ns; r += kDT) {
       const int i = static_cast<int>(in.keys[r] & ((1u << 21) - 1));
       const int dst = nl + r;
@@ -378,7 +381,7 @@ __device__ __forceinline__ void decideBody(const DecideArgs& A) {
       kept += tot;
     }
     __syncthreads();
-    refills = nl - kept;
+    refills += nl - kept;
     nl = kept;
     fromScratch(in, nl);
   }
@@ -475,6 +478,7 @@ __device__ __forceinline__ void decideBody(const DecideArgs& A) {
     A.ctl->n_inter = nl;
     A.ctl->n_ghost = ne - nl;
     A.ctl->n_stage = 0;
+    A.ctl->n_quick = 0;
     A.ctl->n_prev = ne;
     A.ctl->n_far_prev = nfar;
     A.ctl->n_far = 0;
@@ -500,6 +504,96 @@ __device__ __forceinline__ void decideBody(const DecideArgs& A) {
   }
 }
 __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) { decideBody(A); }
+
+// Decide, ahead of the list work: the staged atoms that the recombination
+// loop would put straight back on the site they left (in a cascade, the
+// thousands a step pushed beyond 0.2a that stay within their site's
+// Wigner-Seitz cell).  atom.cpp:28-83 appends such an atom, finds its own
+// vacated site nearest, and refills it with the first list entry that
+// targets it.  The entries that can target a vacated site are the old inter
+// atoms (list indices 0..n_inter-1) and the staged atoms, which follow them
+// in the order of the reference index of the sites they left.  So the staged
+// atom refills its own site exactly when no old inter atom and no staged
+// atom of a lower site index targets that site as well.  Such an atom is put
+// back here with everything decide writes for it (type, id, position,
+// velocity, images, record, the far list); decide skips its stage entry,
+// and its list work, list order and claims come out as before.
+//   kQuickClaim:   a staged atom whose nearest site is its own, with its
+//                  position in the box, opens a claim on that site with its
+//                  priority (kStagedPri + site index);
+//   kQuickCompete: every other entry whose nearest site carries a claim
+//                  lowers it to its own priority (old inter atoms: their
+//                  list index);
+//   kQuickRefill:  a staged atom still holding its site's claim is put back;
+//                  every opened claim is released.
+constexpr int32_t kStagedPri = 1 << 30;
+constexpr int64_t kQuickDone = (int64_t(1) << 42) - 1;  // stage key of an entry put back early
+constexpr int kQuickBlocks = 128;
+
+__device__ __forceinline__ int quickStaged(const InterDev& in, const Ctl* ctl) {
+  return static_cast<int>(min(ctl->n_stage, in.cap));
+}
+__device__ __forceinline__ bool ownSiteNearest(const Geom& G, const InterDev& in, int i) {
+  const double y0 = in.stage_x[0][i], y1 = in.stage_x[1][i], y2 = in.stage_x[2][i];
+  return nearSubBoxSlot(G, y0, y1, y2) == in.stage_slot[i] && outBoxFlags(G, y0, y1, y2) == 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void kQuickClaim(Geom G, InterDev in, const Ctl* ctl) {
+  const int ns = quickStaged(in, ctl);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < ns; i += gridDim.x * kBlock)
+    if (ownSiteNearest(G, in, i))
+      in.site_claim[in.stage_slot[i]] = kStagedPri + static_cast<int32_t>(in.stage_key[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void kQuickCompete(Geom G, InterDev in, const Ctl* ctl) {
+  const int nl = static_cast<int>(ctl->n_inter), ns = quickStaged(in, ctl);
+  for (int q = blockIdx.x * kBlock + threadIdx.x; q < nl + ns; q += gridDim.x * kBlock) {
+    double y0, y1, y2;
+    int32_t pri;
+    int64_t own = -1;
+    if (q < nl) {
+      y0 = in.x[0][q];
+      y1 = in.x[1][q];
+      y2 = in.x[2][q];
+      pri = q;
+    } else {
+      const int i = q - nl;
+      y0 = in.stage_x[0][i];
+      y1 = in.stage_x[1][i];
+      y2 = in.stage_x[2][i];
+      pri = kStagedPri + static_cast<int32_t>(in.stage_key[i]);
+      own = in.stage_slot[i];
+    }
+    const int64_t t = nearSubBoxSlot(G, y0, y1, y2);
+    if (t >= 0 && t != own && in.site_claim[t] != INT_MAX) atomicMin(&in.site_claim[t], pri);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void kQuickRefill(DecideArgs A) {
+  InterDev& in = A.in;
+  const int ns = quickStaged(in, A.ctl);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < ns; i += gridDim.x * kBlock) {
+    if (!ownSiteNearest(A.G, in, i)) continue;
+    const int64_t t = in.stage_slot[i];
+    if (in.site_claim[t] == kStagedPri + static_cast<int32_t>(in.stage_key[i])) {
+      const int8_t ty = static_cast<int8_t>(species(in.stage_type[i]));
+      A.lid[t] = in.stage_id[i];
+      A.ltype[t] = ty;
+      for (int k = 0; k < 3; k++) {
+        A.lx[k][t] = in.stage_x[k][i];
+        A.lv[k][t] = in.stage_v[k][i];
+      }
+      writeLatticeImages(A, t, true, ty);
+      // it left the site by more than 0.2a (that is why it was staged)
+      const unsigned long long q = atomicAdd(reinterpret_cast<unsigned long long*>(&A.ctl->n_far), 1ull);
+      if (q < static_cast<unsigned long long>(in.cap)) in.far_new[q] = t;
+      else setFlag(reinterpret_cast<unsigned long long*>(&A.ctl->err_flags), 1ull);
+      atomicAdd(reinterpret_cast<unsigned long long*>(&A.ctl->n_quick), 1ull);
+      in.stage_key[i] = kQuickDone;
+    }
+    in.site_claim[t] = INT_MAX;
+  }
+}
 
 // Decide, step 3: flag the sub-box sites within the full neighbour list of
 // every entry (kNearInter: their sums visit the inter atoms), and those
@@ -1040,6 +1134,11 @@ void launchDecide(const DevState& s, int mode, bool mixed, hipStream_t st) {
   A.ctl = s.ctl;
   A.skin = s.skin;
   kUnflag<<<kFlagBlocks, kBlock, 0, st>>>(s.G, s.in, s.type, s.ctl);
+  if (mode == 0 && 2 * s.G.plane < kStagedPri) {  // site indices below the staged priority offset
+    kQuickClaim<<<kQuickBlocks, kBlock, 0, st>>>(s.G, s.in, s.ctl);
+    kQuickCompete<<<kQuickBlocks, kBlock, 0, st>>>(s.G, s.in, s.ctl);
+    kQuickRefill<<<kQuickBlocks, kBlock, 0, st>>>(A);
+  }
   kDecide<<<1, kDT, 0, st>>>(A);
   kFlag<<<kFlagBlocks, kBlock, 0, st>>>(s.G, s.in, s.type, s.ctl);
 }
