@@ -273,6 +273,13 @@ __global__ __launch_bounds__(kBlock) void kUnflag(Geom G, InterDev in, int8_t* l
              static_cast<int64_t>(gridDim.x) * kBlock);
 }
 
+// Priority of a staged atom's claim on a vacated site (above every list
+// index, ordered by the reference index of the site it left), and the stage
+// key of an entry that kQuickRefill has put back.
+constexpr int32_t kStagedPri = 1 << 30;
+constexpr int64_t kQuickDone = (int64_t(1) << 42) - 1;
+constexpr int kQuickBlocks = 128;
+
 // Decide, step 2: the list work in one workgroup (the lists hold the
 // interstitials and the atoms that moved more than 0.2a this step: hundreds
 // to thousands in a developed cascade).
@@ -296,20 +303,25 @@ __device__ __forceinline__ void decideBody(const DecideArgs& A) {
     // P1: append the new inter atoms in the order of the sites they left
     // (atom.cpp:28-53 walks the sub-box z -> y -> x and appends): sort the
     // staged atoms by the reference linear index of their site.  Entries
-    // that kQuickRefill already put back carry kQuickDone and sort last.
+    // that kQuickRefill already put back (key kQuickDone) are left out.
     const int nsAll = static_cast<int>(min(A.ctl->n_stage, in.cap));
-    const int nq = static_cast<int>(A.ctl->n_quick);
-    int ns = nsAll - nq;
+    staged = nsAll;
+    refills = static_cast<int>(A.ctl->n_quick);
+    int ns = 0;  // the entries still staged, compacted before the sort
+    for (int base = 0; base < nsAll; base += kDT) {
+      const int i = base + tid;
+      const int live = (i < nsAll && in.stage_key[i] != kQuickDone) ? 1 : 0;
+      int tot;
+      const int ex = blockScan(live, tot);
+      if (live) in.keys[ns + ex] = (static_cast<uint64_t>(in.stage_key[i]) << 21) | static_cast<uint64_t>(i);
+      ns += tot;
+    }
+    __syncthreads();
+    sortKeys(in.keys, ns, sKeys);
     if (nl + ns > in.cap) {
       if (tid == 0) setFlag(err, 1ull);
       ns = static_cast<int>(in.cap) - nl;
     }
-    staged = nsAll;
-    refills = nq;
-    for (int i = tid; i < nsAll; i += kDT)
-      in.keys[i] = (static_cast<uint64_t>(in.stage_key[i]) << 21) | static_cast<uint64_t>(i);
-    __syncthreads();
-    sortKeys(in.keys, nsAll, sKeys);
     for (int r = tid; r < ns; r += kDT) {
       const int i = static_cast<int>(in.keys[r] & ((1u << 21) - 1));
       const int dst = nl + r;
@@ -526,9 +538,6 @@ __global__ __launch_bounds__(kDT) void kDecide(DecideArgs A) { decideBody(A); }
 //                  list index);
 //   kQuickRefill:  a staged atom still holding its site's claim is put back;
 //                  every opened claim is released.
-constexpr int32_t kStagedPri = 1 << 30;
-constexpr int64_t kQuickDone = (int64_t(1) << 42) - 1;  // stage key of an entry put back early
-constexpr int kQuickBlocks = 128;
 
 __device__ __forceinline__ int quickStaged(const InterDev& in, const Ctl* ctl) {
   return static_cast<int>(min(ctl->n_stage, in.cap));

@@ -35,19 +35,23 @@ namespace {
 
 using namespace site;
 
-// firststep from stored forces + lattice decide + position halo, in a grid
-// that strides over the virtual blocks of a site launch (a whole number of
-// blocks per XCD keeps each on its XCD's slab), so that few wavefronts add
-// their largest drift to the one word every wavefront of the pass feeds.
+// firststep from stored forces + lattice decide + position halo.  A
+// streaming pass (no site reads another's data), so its grid walks the sites
+// in slot order, x fastest, for long runs of consecutive addresses in every
+// array; it strides over them with a few blocks per compute unit, so that few
+// wavefronts add their largest drift to the one word every wavefront of the
+// pass feeds.
 __global__ __launch_bounds__(kBlock) void kFirst(Geom G, CXPtr x, XPtr xn, const int8_t* __restrict__ type,
                                                  XPtr v, CXPtr f, const uint64_t* __restrict__ id, VerletParams vp,
-                                                 StageArgs st, Ctl* ctl, float4* __restrict__ recn, uint32_t nb) {
+                                                 StageArgs st, Ctl* ctl, float4* __restrict__ recn) {
 #pragma clang fp contract(off)  // the half kick and drift round as newton_motion.cpp does
   unsigned long long db = 0;  // |dx|^2 bits of the drift, largest over the lane's sites
-  for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+  const int64_t nsites = 2 * G.nsub;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; t < nsites;
+       t += static_cast<int64_t>(gridDim.x) * kBlock) {
     int64_t s;
     int p, cx, cy, cz;
-    if (!siteOfBlock(G, b, threadIdx.x, s, p, cx, cy, cz)) continue;
+    siteInOrder(G, t, s, p, cx, cy, cz);
     const int8_t ti = type[s];
     double y0 = x.c[0][s], y1 = x.c[1][s], y2 = x.c[2][s];
     if (!isVacant(ti)) {
@@ -316,10 +320,9 @@ void launchPackRec(const DevState& s, int buf, hipStream_t st) {
 }
 
 void launchFirst(const DevState& s, bool mixed, const VerletParams& vp, hipStream_t st) {
-  const unsigned nb = siteGrid(s.G);
   kFirst<<<strideGrid(s.G, s.ncu), kBlock, 0, st>>>(s.G, cxp(s.x[s.cur]), xp(s.x[1 - s.cur]), s.type, xp(s.v),
                                                     cxp(s.f), s.id, vp, stageOf(s), s.ctl,
-                                                    mixed ? s.rec[1 - s.cur] : nullptr, nb);
+                                                    mixed ? s.rec[1 - s.cur] : nullptr);
 }
 
 int64_t launchThermo(const DevState& s, hipStream_t st) {

@@ -47,6 +47,18 @@ __device__ __forceinline__ bool siteOfBlock(const Geom& G, uint32_t b, uint32_t 
   s = slotOf(G, p, cx, cy, cz);
   return true;
 }
+// Site t of the plain slot order (t < 2 nsub): parity, then z, y, x with x
+// fastest, so consecutive t are consecutive slots along a row.
+__device__ __forceinline__ void siteInOrder(const Geom& G, int64_t t, int64_t& s, int& p, int& cx, int& cy,
+                                            int& cz) {
+  p = t >= G.nsub ? 1 : 0;
+  const uint32_t r = static_cast<uint32_t>(t - p * G.nsub), n0 = static_cast<uint32_t>(G.n0);
+  const uint32_t row = r / n0, n1 = static_cast<uint32_t>(G.n1);
+  cx = static_cast<int>(r - row * n0);
+  cz = static_cast<int>(row / n1);
+  cy = static_cast<int>(row - static_cast<uint32_t>(cz) * n1);
+  s = slotOf(G, p, cx, cy, cz);
+}
 __device__ __forceinline__ bool siteOfThread(const Geom& G, int64_t& s, int& p, int& cx, int& cy, int& cz) {
   return siteOfBlock(G, blockIdx.x, threadIdx.x, s, p, cx, cy, cz);
 }
@@ -226,7 +238,17 @@ struct StageArgs {
 __device__ __forceinline__ void stageInter(const StageArgs& st, Ctl* ctl, int64_t key, int64_t slot, uint64_t id,
                                            int8_t type, double x0, double x1, double x2, double v0, double v1,
                                            double v2) {
-  const unsigned long long k = atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->n_stage), 1ull);
+  // one counter update per wavefront: the lanes that stage (the active ones
+  // here) take consecutive entries; the decide orders the entries by key
+  const uint64_t lanes = __ballot(1);
+  const int leader = __builtin_ctzll(lanes);
+  const int rank = __builtin_popcountll(lanes & ((1ull << (threadIdx.x & 63)) - 1ull));
+  unsigned long long base = 0;
+  if (static_cast<int>(threadIdx.x & 63) == leader)
+    base = atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->n_stage),
+                     static_cast<unsigned long long>(__builtin_popcountll(lanes)));
+  base = __shfl(base, leader, 64);
+  const unsigned long long k = base + static_cast<unsigned long long>(rank);
   if (static_cast<int64_t>(k) >= st.cap) {
     atomicOr(reinterpret_cast<unsigned long long*>(&ctl->err_flags), 1ull);
     return;
