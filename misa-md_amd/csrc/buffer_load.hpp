@@ -81,6 +81,9 @@ __device__ __forceinline__ f32x3 ld3(Rsrc r, uint32_t o) {
 __device__ __forceinline__ float4 ldRec4(Rsrc r, uint32_t v, uint32_t s) {
   return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, 0));
 }
+__device__ __forceinline__ double2 ldRec64(Rsrc r, uint32_t v, uint32_t s) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, 0));
+}
 
 }  // namespace site
 }  // namespace misa

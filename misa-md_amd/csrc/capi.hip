@@ -428,6 +428,10 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
   if (rc == MISA_OK) rc |= devAlloc(h, &s.df64, slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.rec[0], slots);
   if (rc == MISA_OK) rc |= devAlloc(h, &s.rec[1], slots);
+  for (int b = 0; b < 2 && rc == MISA_OK; b++) {
+    rc |= devAlloc(h, &s.pxy[b], slots);
+    rc |= devAlloc(h, &s.pzw[b], slots);
+  }
   if (rc == MISA_OK) rc |= devAlloc(h, &s.dl, 4 * kMaxLatOffsets);
   // a balanced slab may hold up to twice the planes of an even cut
   h->bpx_cap = 2 * G.W.bpx;
@@ -941,7 +945,7 @@ int misa_compute_forces(misa_handle* h) {
   const bool prof = h->profile;
   h->profile = false;
   const bool mixed = h->precision == MISA_PREC_MIXED;
-  if (mixed) launchPackRec(h->s, h->s.cur, h->stream);
+  launchPackRec(h->s, h->s.cur, !mixed, h->stream);
   launchFarScan(h->s, h->stream);
   launchDecide(h->s, 1, mixed, h->stream);
   forcePass(h, 2, 2, true, vp);
@@ -1185,7 +1189,7 @@ int misa_thermo_compute(misa_handle* h, misa_thermo* out) {
   const uint64_t dof = 3 * n - 3;
   out->temperature = out->mvv * kMvv2e / (dof * kBoltz);
   if (h->have_pot && h->have_offsets && h->forces_ready) {
-    const int64_t ne = launchEnergy(h->s, h->lat, h->t64, h->stream);
+    const int64_t ne = launchEnergy(h->s, h->lat, h->t64, h->precision == MISA_PREC_MIXED, h->stream);
     HIPCHECK(h, hipGetLastError());
     std::vector<double> q(static_cast<size_t>(3 * ne));
     HIPCHECK(h, hipMemcpyAsync(q.data(), h->s.partials, q.size() * 8, hipMemcpyDeviceToHost, h->stream));

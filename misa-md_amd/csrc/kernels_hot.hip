@@ -159,7 +159,8 @@ template <typename T, int ACC, int U>
 __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T> tb, CXPtr x,
                                                  const int8_t* __restrict__ type, T* __restrict__ df,
                                                  double* __restrict__ rho_out, uint64_t* __restrict__ wmask,
-                                                 uint64_t* __restrict__ smask, double rcs2, InterView iv) {
+                                                 uint64_t* __restrict__ smask, double rcs2, InterView iv,
+                                                 Rec64 r64) {
   int64_t s;
   int p, cx, cy, cz;
   if (!siteOfThread(G, s, p, cx, cy, cz)) return;
@@ -173,38 +174,36 @@ __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T>
   uint64_t w0 = 0, w1 = 0, s0 = 0, s1 = 0;
   if (!isVacant(ti)) {
     const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
-    const Rsrc rx0 = rsrcBelow(x.c[0], O.lo), rx1 = rsrcBelow(x.c[1], O.lo), rx2 = rsrcBelow(x.c[2], O.lo);
-    const Rsrc rt = rsrcBelow(type, O.lo);
+    // neighbours from the FP64 site records (a vacancy sits far out, so every
+    // distance test against it fails as the type test did)
+    const Rsrc rxy = rsrcBelow(r64.xy, O.lo), rzw = rsrcBelow(r64.zw, O.lo);
     const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoV));
-    const uint32_t o8 = static_cast<uint32_t>(s) * 8u, o1 = static_cast<uint32_t>(s);
+    const uint32_t o16 = static_cast<uint32_t>(s) * 16u;
     const uint32_t* sh = O.sh[p];
     const int nk = O.n[p];
     Sum1<ACC> acc;
     auto group = [&](auto UC, const int* ks) {
       constexpr int V = decltype(UC)::value;
-      int8_t tj[V];
-      double xj0[V], xj1[V], xj2[V];
+      double2 xy[V], zw[V];
 #pragma unroll
       for (int u = 0; u < V; ++u) {
-        const uint32_t so = sh[ks[u]];
-        tj[u] = ldB(rt, o1, so);
-        xj0[u] = ldD(rx0, o8, so * 8u);
-        xj1[u] = ldD(rx1, o8, so * 8u);
-        xj2[u] = ldD(rx2, o8, so * 8u);
+        const uint32_t so = sh[ks[u]] * 16u;
+        xy[u] = ldRec64(rxy, o16, so);
+        zw[u] = ldRec64(rzw, o16, so);
       }
       bool in[V];
       T pp[V];
       T rec[V][kRhoRec];
 #pragma unroll
       for (int u = 0; u < V; ++u) {
-        const double r2 = dist2D(xi0 - xj0[u], xi1 - xj1[u], xi2 - xj2[u]);
-        in[u] = !isVacant(tj[u]) && r2 < G.rc2;
-        if (!skin && __ballot(!isVacant(tj[u]) && r2 < rcs2)) setBit(s0, s1, ks[u]);
+        const double r2 = dist2D(xi0 - xy[u].x, xi1 - xy[u].y, xi2 - zw[u].x);
+        in[u] = r2 < G.rc2;
+        if (!skin && __ballot(r2 < rcs2)) setBit(s0, s1, ks[u]);
         if (__ballot(in[u])) setBit(w0, w1, ks[u]);
         if (in[u]) {
           const T q2 = static_cast<T>(r2);
           const int m = knotOf(q2 * rsqT(q2), tb.rdr, tb.nr, false, pp[u]);
-          const uint32_t off = (static_cast<uint32_t>(species(tj[u]) * tb.nr + m)) *
+          const uint32_t off = (static_cast<uint32_t>(recSpecies64(zw[u].y) * tb.nr + m)) *
                                static_cast<uint32_t>(kRhoRec * sizeof(T));
           ldRec<T, kRhoRec>(rtab, off, rec[u]);
         }
@@ -253,14 +252,22 @@ __global__ __launch_bounds__(kBlock) void kRhoDf(Geom G, LatOffsets O, Tables<T>
   }
   writeMasks(!isVacant(ti), wv, w0, w1, wmask, !skin, s0, s1, smask);
   df[s] = dfi;
+  // F' into the w of the site's record (and of its images' records)
+  double* w64 = reinterpret_cast<double*>(r64.zw) + 1;
+  const double wi = packW64(static_cast<double>(dfi), species(ti));
+  if (!isVacant(ti)) w64[2 * s] = wi;
   if (rho_out) {
     rho_out[s] = rho;
     forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) {
       df[q] = dfi;
       rho_out[q] = rho;
+      if (!isVacant(ti)) w64[2 * q] = wi;
     });
   } else {
-    forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) { df[q] = dfi; });
+    forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) {
+      df[q] = dfi;
+      if (!isVacant(ti)) w64[2 * q] = wi;
+    });
   }
 }
 
@@ -271,7 +278,7 @@ __device__ __forceinline__ void verletTail(const Geom& G, int64_t s, int p, int 
                                            int ei, double xi0, double xi1, double xi2, double f0, double f1,
                                            double f2, XPtr xn, XPtr v, XPtr f, const uint64_t* __restrict__ id,
                                            const VerletParams& vp, const StageArgs& st, Ctl* ctl,
-                                           unsigned long long& vb, unsigned long long& db) {
+                                           const Rec64& rn64, unsigned long long& vb, unsigned long long& db) {
 #pragma clang fp contract(off)
   const StepDt sd = stepDt(vp, ei);
   const double c = sd.c;
@@ -311,10 +318,13 @@ __device__ __forceinline__ void verletTail(const Geom& G, int64_t s, int p, int 
   xn.c[0][s] = y0;
   xn.c[1][s] = y1;
   xn.c[2][s] = y2;
+  putRec64(rn64, s, ti, y0, y1, y2);
   forEachImage(G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
-    xn.c[0][q] = y0 + sx * G.L0;
-    xn.c[1][q] = y1 + sy * G.L1;
-    xn.c[2][q] = y2 + sz * G.L2;
+    const double q0 = y0 + sx * G.L0, q1 = y1 + sy * G.L1, q2 = y2 + sz * G.L2;
+    xn.c[0][q] = q0;
+    xn.c[1][q] = q1;
+    xn.c[2][q] = q2;
+    putRec64(rn64, q, ti, q0, q1, q2);
   });
 }
 
@@ -329,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
                                                  const int8_t* __restrict__ type, const T* __restrict__ df,
                                                  const uint64_t* __restrict__ wmask, XPtr v, XPtr f,
                                                  const uint64_t* __restrict__ id, VerletParams vp, StageArgs st,
-                                                 Ctl* ctl, InterView iv) {
+                                                 Ctl* ctl, InterView iv, Rec64 r64, Rec64 rn64) {
   unsigned long long vb = 0;  // |v|^2 bits after the secondstep (MODE 0)
   unsigned long long db = 0;  // |dx|^2 bits of the drift (MODE 1)
   [&] {
@@ -347,37 +357,38 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
       xn.c[0][s] = xi0;
       xn.c[1][s] = xi1;
       xn.c[2][s] = xi2;
+      putRec64(rn64, s, ti, xi0, xi1, xi2);
       forEachImage(G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
         xn.c[0][q] = xi0 + sx * G.L0;
         xn.c[1][q] = xi1 + sy * G.L1;
         xn.c[2][q] = xi2 + sz * G.L2;
+        putRec64(rn64, q, ti, xn.c[0][q], xn.c[1][q], xn.c[2][q]);
       });
     }
     return;
   }
   const int ei = species(ti);
   const T dfi = df[s];
-  const Rsrc rx0 = rsrcBelow(x.c[0], O.lo), rx1 = rsrcBelow(x.c[1], O.lo), rx2 = rsrcBelow(x.c[2], O.lo);
-  const Rsrc rt = rsrcBelow(type, O.lo);
-  const Rsrc rdf = rsrcBelow(df, O.lo);
+  // neighbours from the FP64 site records: position and, in w, F' with the
+  // species (a vacancy sits far out, so the distance test drops it)
+  const Rsrc rxy = rsrcBelow(r64.xy, O.lo), rzw = rsrcBelow(r64.zw, O.lo);
   const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairF));
   const Rsrc rtabR = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoV));
-  const uint32_t o8 = static_cast<uint32_t>(s) * 8u, o1 = static_cast<uint32_t>(s);
-  const uint32_t oT = static_cast<uint32_t>(s) * static_cast<uint32_t>(sizeof(T));
+  const uint32_t o16 = static_cast<uint32_t>(s) * 16u;
   const uint32_t pairBase = static_cast<uint32_t>(ei * 3) * static_cast<uint32_t>(tb.nr);
   const uint32_t* sh = O.sh[p];
   Sum3<ACC> acc;
   auto group = [&](auto UC, const int* ks) {
     constexpr int V = decltype(UC)::value;
-    int8_t tj[V];
-    double d0[V], d1[V], d2[V];
+    double d0[V], d1[V], d2[V], wj[V];
 #pragma unroll
     for (int u = 0; u < V; ++u) {
-      const uint32_t so = sh[ks[u]];
-      tj[u] = ldB(rt, o1, so);
-      d0[u] = ldD(rx0, o8, so * 8u);
-      d1[u] = ldD(rx1, o8, so * 8u);
-      d2[u] = ldD(rx2, o8, so * 8u);
+      const uint32_t so = sh[ks[u]] * 16u;
+      const double2 xy = ldRec64(rxy, o16, so), zw = ldRec64(rzw, o16, so);
+      d0[u] = xy.x;
+      d1[u] = xy.y;
+      d2[u] = zw.x;
+      wj[u] = zw.y;
     }
     bool in[V];
     T pp[V], ri[V], dfj[V];
@@ -388,13 +399,12 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
       d1[u] = xi1 - d1[u];
       d2[u] = xi2 - d2[u];
       const double r2 = dist2D(d0[u], d1[u], d2[u]);
-      in[u] = !isVacant(tj[u]) && r2 < G.rc2;
+      in[u] = r2 < G.rc2;
       if (in[u]) {
-        const uint32_t so = sh[ks[u]];
         const T q2 = static_cast<T>(r2);
         ri[u] = rsqT(q2);
         const int m = knotOf(q2 * ri[u], tb.rdr, tb.nr, false, pp[u]);
-        const int ej = species(tj[u]);
+        const int ej = recSpecies64(wj[u]);
         const uint32_t off = (pairBase + static_cast<uint32_t>(ej * tb.nr + m)) *
                              static_cast<uint32_t>(kPairRec * sizeof(T));
         ldRec<T, kPairRec>(rtab, off, rec[u]);
@@ -405,7 +415,7 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
 #pragma unroll
           for (int q = 0; q < kRhoRec; q++) reci[u][q] = rec[u][4 + q];
         }
-        dfj[u] = ldT<T>(rdf, oT, so * static_cast<uint32_t>(sizeof(T)));
+        dfj[u] = static_cast<T>(wj[u]);
       }
     }
 #pragma unroll
@@ -454,7 +464,7 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
   double f0, f1, f2;
   sum3Value<ACC>(acc, f0, f1, f2);
   checkFinite(ctl, f0, f1, f2);
-  verletTail<MODE>(G, s, p, cx, cy, cz, ti, ei, xi0, xi1, xi2, f0, f1, f2, xn, v, f, id, vp, st, ctl, vb, db);
+  verletTail<MODE>(G, s, p, cx, cy, cz, ti, ei, xi0, xi1, xi2, f0, f1, f2, xn, v, f, id, vp, st, ctl, rn64, vb, db);
   }();
   if (MODE == 0 && vp.vmax) waveMaxTo(vb, vp.vmax);
   if (MODE == 1) waveMaxTo(db, &ctl->dispmax2);
@@ -818,7 +828,7 @@ void launchRhoDf(const DevState& s, const LatOffsets& o, const Tables<double>& t
   const double rcs = std::sqrt(s.G.rc2) + s.skin;
   if (!mixed)
     kRhoDf<double, 0, 4><<<grid, kBlock, 0, st>>>(s.G, o, t64, x, s.type, s.df64, rho, s.wmask, s.smask, rcs * rcs,
-                                                   iv);
+                                                   iv, Rec64{s.pxy[s.cur], s.pzw[s.cur]});
   else {
     const float rcs2 = static_cast<float>(rcs * rcs);
 #define MISA_RHO_M(A, U)                                                                                   \
@@ -849,9 +859,11 @@ int launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t6
   const InterView iv = interView(s, mixed);
   const bool lds = mixed && useLds(s);
   const LdsKnots lk = ldsKnotsForce(t32.nr);
-#define MISA_FORCE(M) \
+#define MISA_FORCE(M)                                                                                       \
   kForce<double, 0, M, kUDouble><<<grid, kBlock, 0, st>>>(s.G, o, t64, x, xn, s.type, s.df64, s.wmask, xp(s.v), \
-                                                         xp(s.f), s.id, vp, sa, s.ctl, iv)
+                                                         xp(s.f), s.id, vp, sa, s.ctl, iv,                   \
+                                                         Rec64{s.pxy[s.cur], s.pzw[s.cur]},                  \
+                                                         Rec64{s.pxy[1 - s.cur], s.pzw[1 - s.cur]})
 #define MISA_FORCE_MU(A, M, U)                                                                                \
   do {                                                                                                        \
     static const bool ldsReady =                                                                              \

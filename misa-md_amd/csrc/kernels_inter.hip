@@ -52,6 +52,7 @@ struct DecideArgs {
   int8_t* ltype;
   uint64_t* lid;
   float4* lrec;  // mixed precision: site records of the current buffer, else null
+  Rec64 lrec64;  // FP64: site records of the current buffer, else null
   InterDev in;
   Ctl* ctl;
   double skin;  // neighbour skin of the mixed density pass (0: off)
@@ -146,6 +147,7 @@ __device__ __forceinline__ void writeLatticeImages(const DecideArgs& A, int64_t 
     if (!isVacant(t)) r = siteRec(A.G, p, cx, cy, cz, y0, y1, y2, species(t));
     A.lrec[slot] = r;
   }
+  if (A.lrec64.xy) putRec64(A.lrec64, slot, t, y0, y1, y2);
   forEachImage(A.G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
     A.ltype[q] = t;
     if (pos) {
@@ -154,6 +156,7 @@ __device__ __forceinline__ void writeLatticeImages(const DecideArgs& A, int64_t 
       A.lx[2][q] = y2 + sz * A.G.L2;
     }
     if (A.lrec) A.lrec[q] = r;
+    if (A.lrec64.xy) putRec64(A.lrec64, q, t, A.lx[0][q], A.lx[1][q], A.lx[2][q]);
   });
 }
 
@@ -1081,6 +1084,7 @@ void fillDecideArgs(const BatchBox& b, int c, void* out) {
   A.ltype = b.type;
   A.lid = b.id;
   A.lrec = b.rec[c];
+  A.lrec64 = Rec64{nullptr, nullptr};
   A.in = b.in;
   A.ctl = b.ctl;
   A.skin = b.skin;
@@ -1139,6 +1143,7 @@ void launchDecide(const DevState& s, int mode, bool mixed, hipStream_t st) {
   A.ltype = s.type;
   A.lid = s.id;
   A.lrec = mixed ? s.rec[s.cur] : nullptr;
+  A.lrec64 = mixed ? Rec64{nullptr, nullptr} : Rec64{s.pxy[s.cur], s.pzw[s.cur]};
   A.in = s.in;
   A.ctl = s.ctl;
   A.skin = s.skin;

@@ -43,7 +43,7 @@ using namespace site;
 // pass feeds.
 __global__ __launch_bounds__(kBlock) void kFirst(Geom G, CXPtr x, XPtr xn, const int8_t* __restrict__ type,
                                                  XPtr v, CXPtr f, const uint64_t* __restrict__ id, VerletParams vp,
-                                                 StageArgs st, Ctl* ctl, float4* __restrict__ recn) {
+                                                 StageArgs st, Ctl* ctl, float4* __restrict__ recn, Rec64 rn64) {
 #pragma clang fp contract(off)  // the half kick and drift round as newton_motion.cpp does
   unsigned long long db = 0;  // |dx|^2 bits of the drift, largest over the lane's sites
   const int64_t nsites = 2 * G.nsub;
@@ -86,8 +86,21 @@ __global__ __launch_bounds__(kBlock) void kFirst(Geom G, CXPtr x, XPtr xn, const
       recn[s] = r;
       forEachImage(G, p, cx, cy, cz, [&](int64_t q, int, int, int) { recn[q] = r; });
     }
+    if (rn64.xy) {
+      putRec64(rn64, s, ti, y0, y1, y2);
+      forEachImage(G, p, cx, cy, cz, [&](int64_t q, int sx, int sy, int sz) {
+        putRec64(rn64, q, ti, y0 + sx * G.L0, y1 + sy * G.L1, y2 + sz * G.L2);
+      });
+    }
   }
   waveMaxTo(db, &ctl->dispmax2);
+}
+
+// FP64 site records from x and type over every slot, ghosts included.
+__global__ __launch_bounds__(kBlock) void kPackRec64(Geom G, CXPtr x, const int8_t* __restrict__ type, Rec64 r) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= 2 * G.plane) return;
+  putRec64(r, t, type[t], x.c[0][t], x.c[1][t], x.c[2][t]);
 }
 
 // Site records from x and type over every slot, ghosts included.
@@ -243,11 +256,27 @@ __global__ __launch_bounds__(kBlock) void kThermo(Geom G, CXPtr v, CXPtr f, cons
   if (threadIdx.x == 0) partials[7 * blockIdx.x + 6] = vmax[0];
 }
 
+// F'(rho) with the FP64 tables for every occupied slot, ghosts included,
+// from the stored densities: what the FP64 density pass leaves in df, for
+// the energy after a mixed-precision step.
+__global__ __launch_bounds__(kBlock) void kDfOfRho(Geom G, Tables<double> tb, const int8_t* __restrict__ type,
+                                                   const double* __restrict__ rho, double* __restrict__ df) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= 2 * G.plane) return;
+  const int8_t ti = type[t];
+  if (!isVacant(ti)) df[t] = dEmbed(tb, species(ti), rho[t]);
+}
+
 // Potential energy and virial with FP64 tables over the full list:
-// per site F(rho_i) + 1/2 sum_j phi(r_ij) and 1/2 sum_j r_ij^2 fpair_ij.
+// per site F(rho_i) + 1/2 sum_j phi(r_ij) and 1/2 sum_j r_ij^2 fpair_ij, with
+// F'(rho_j) read from df (dEmbed of the stored densities).  The pruned list
+// goes in groups of four whose loads are all issued before the first pair
+// term; near a far lattice atom the rest of the full list follows one by
+// one.  The terms are added in list order.
 __global__ __launch_bounds__(kBlock) void kEnergy(Geom G, LatOffsets O, const FullPlaneOffsets* __restrict__ full,
                                                   Tables<double> tb, CXPtr x, const int8_t* __restrict__ type,
-                                                  const double* __restrict__ rho, double* partials) {
+                                                  const double* __restrict__ rho, const double* __restrict__ df,
+                                                  double* partials) {
   int64_t s;
   int p, cx, cy, cz;
   double vals[3] = {0, 0, 0};
@@ -259,13 +288,8 @@ __global__ __launch_bounds__(kBlock) void kEnergy(Geom G, LatOffsets O, const Fu
       const double dfi = dEmbed(tb, ei, ri);
       vals[0] = embed(tb, ei, ri);
       const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
-      // the pruned list, and near a far lattice atom the rest of the full list
-      const int nk = O.n[p] + ((ti & kWide) ? full->nw[p] : 0);
-      for (int k = 0; k < nk; ++k) {
-        const int64_t j = s + (k < O.n[p] ? O.off[p][k] : full->woff[p][k - O.n[p]]);
-        const int8_t tj = type[j];
-        if (k >= O.n[p] && !wideTarget(ti, tj)) continue;
-        const double d0 = xi0 - x.c[0][j], d1 = xi1 - x.c[1][j], d2 = xi2 - x.c[2][j];
+      auto pair = [&](int8_t tj, double xj0, double xj1, double xj2, double dfj) {
+        const double d0 = xi0 - xj0, d1 = xi1 - xj1, d2 = xi2 - xj2;
         const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
         if (!isVacant(tj) && r2 < G.rc2) {
           const int ej = species(tj);
@@ -273,8 +297,37 @@ __global__ __launch_bounds__(kBlock) void kEnergy(Geom G, LatOffsets O, const Fu
           double q;
           const int m = knotOf(r, tb.rdr, tb.nr, false, q);
           vals[1] += 0.5 * splineVal(tb.z2 + pairIdx(ei, ej) * tb.sR + static_cast<int64_t>(m) * kKnot, q) / r;
-          vals[2] += 0.5 * r2 * pairForce(tb, ei, ej, r2, dfi, dEmbed(tb, ej, rho[j]));
+          vals[2] += 0.5 * r2 * pairForce(tb, ei, ej, r2, dfi, dfj);
         }
+      };
+      constexpr int V = 4;
+      const int np = O.n[p];
+      int k = 0;
+      for (; k + V <= np; k += V) {
+        int8_t tj[V];
+        double xj0[V], xj1[V], xj2[V], dfj[V];
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+          const int64_t j = s + O.off[p][k + u];
+          tj[u] = type[j];
+          xj0[u] = x.c[0][j];
+          xj1[u] = x.c[1][j];
+          xj2[u] = x.c[2][j];
+          dfj[u] = df[j];
+        }
+#pragma unroll
+        for (int u = 0; u < V; ++u) pair(tj[u], xj0[u], xj1[u], xj2[u], dfj[u]);
+      }
+      for (; k < np; ++k) {
+        const int64_t j = s + O.off[p][k];
+        pair(type[j], x.c[0][j], x.c[1][j], x.c[2][j], df[j]);
+      }
+      // near a far lattice atom, the rest of the full list
+      const int nw = (ti & kWide) ? full->nw[p] : 0;
+      for (int w = 0; w < nw; ++w) {
+        const int64_t j = s + full->woff[p][w];
+        const int8_t tj = type[j];
+        if (wideTarget(ti, tj)) pair(tj, x.c[0][j], x.c[1][j], x.c[2][j], df[j]);
       }
     }
   }
@@ -314,15 +367,20 @@ void launchDownload(const DevState& s, AtomElement* aos_dev, int64_t sx, int64_t
                                      reinterpret_cast<const float*>(s.rec[s.cur]) + 3);
 }
 
-void launchPackRec(const DevState& s, int buf, hipStream_t st) {
+void launchPackRec(const DevState& s, int buf, bool fp64, hipStream_t st) {
   const unsigned grid = static_cast<unsigned>((2 * s.G.plane + kBlock - 1) / kBlock);
-  kPackRec<<<grid, kBlock, 0, st>>>(s.G, cxp(s.x[buf]), s.type, s.rec[buf]);
+  if (fp64)
+    kPackRec64<<<grid, kBlock, 0, st>>>(s.G, cxp(s.x[buf]), s.type, Rec64{s.pxy[buf], s.pzw[buf]});
+  else
+    kPackRec<<<grid, kBlock, 0, st>>>(s.G, cxp(s.x[buf]), s.type, s.rec[buf]);
 }
 
 void launchFirst(const DevState& s, bool mixed, const VerletParams& vp, hipStream_t st) {
   kFirst<<<strideGrid(s.G, s.ncu), kBlock, 0, st>>>(s.G, cxp(s.x[s.cur]), xp(s.x[1 - s.cur]), s.type, xp(s.v),
                                                     cxp(s.f), s.id, vp, stageOf(s), s.ctl,
-                                                    mixed ? s.rec[1 - s.cur] : nullptr);
+                                                    mixed ? s.rec[1 - s.cur] : nullptr,
+                                                    mixed ? Rec64{nullptr, nullptr}
+                                                          : Rec64{s.pxy[1 - s.cur], s.pzw[1 - s.cur]});
 }
 
 int64_t launchThermo(const DevState& s, hipStream_t st) {
@@ -331,9 +389,14 @@ int64_t launchThermo(const DevState& s, hipStream_t st) {
   return grid;
 }
 
-int64_t launchEnergy(const DevState& s, const LatOffsets& o, const Tables<double>& t64, hipStream_t st) {
+int64_t launchEnergy(const DevState& s, const LatOffsets& o, const Tables<double>& t64, bool mixed,
+                     hipStream_t st) {
+  if (mixed) {  // the FP64 density pass has not left F' in df
+    const unsigned g = static_cast<unsigned>((2 * s.G.plane + kBlock - 1) / kBlock);
+    kDfOfRho<<<g, kBlock, 0, st>>>(s.G, t64, s.type, s.rho, s.df64);
+  }
   const unsigned grid = siteGrid(s.G);
-  kEnergy<<<grid, kBlock, 0, st>>>(s.G, o, s.in.full, t64, cxp(s.x[s.cur]), s.type, s.rho, s.partials);
+  kEnergy<<<grid, kBlock, 0, st>>>(s.G, o, s.in.full, t64, cxp(s.x[s.cur]), s.type, s.rho, s.df64, s.partials);
   return grid;
 }
 

@@ -73,6 +73,10 @@ struct DevState {
   // shift * 16 in w (2 x kMaxLatOffsets)
   float4* rec[2] = {nullptr, nullptr};
   float4* dl = nullptr;
+  // FP64 precision: site records per slot ({x, y}, {z, w}; ping-pong with
+  // x), site_common.hpp Rec64
+  double2* pxy[2] = {nullptr, nullptr};
+  double2* pzw[2] = {nullptr, nullptr};
   // per wavefront of the site kernels: the pruned offsets within the cutoff
   // for some lane, from the mixed density pass for the force pass (2 words)
   uint64_t* wmask = nullptr;
@@ -122,12 +126,14 @@ void launchRhoDf(const DevState& s, const LatOffsets& o, const Tables<double>& t
 int launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t64, const Tables<float>& t32,
                 bool mixed, int acc, int mode, const VerletParams& vp, hipStream_t st);
 void launchFirst(const DevState& s, bool mixed, const VerletParams& vp, hipStream_t st);
-// Site records of buffer `buf` from x and type over every slot (mixed mode).
-void launchPackRec(const DevState& s, int buf, hipStream_t st);
+// Site records of buffer `buf` from x and type over every slot (the mixed
+// records, or with fp64 the FP64 records).
+void launchPackRec(const DevState& s, int buf, bool fp64, hipStream_t st);
 // Partial sums per block: [mvv, n_lat, n_vac, fx, fy, fz, vmax2] (7 per block)
 int64_t launchThermo(const DevState& s, hipStream_t st);
 // Partial sums per block: [e_embed, e_pair, virial] (3 per block)
-int64_t launchEnergy(const DevState& s, const LatOffsets& o, const Tables<double>& t64, hipStream_t st);
+int64_t launchEnergy(const DevState& s, const LatOffsets& o, const Tables<double>& t64, bool mixed,
+                     hipStream_t st);
 void launchScaleV(const DevState& s, double factor, hipStream_t st);
 // Adaptive step: max |v|^2 over lattice and inter atoms into Ctl::vmax2, then
 // Ctl::dt = min(dt_max, max_disp / max|v|) and Ctl::t_sim += Ctl::dt; with

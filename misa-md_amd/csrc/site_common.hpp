@@ -178,6 +178,36 @@ __device__ __forceinline__ float4 siteRec(const Geom& G, int p, int cx, int cy, 
   return packRec(x0 - X, x1 - Y, x2 - Z, sp, 0.0f);
 }
 
+// Site records of the FP64 passes: per slot {x, y} and {z, w} in two arrays
+// of 16-byte halves (a neighbour costs two 16-byte loads, at the offsets the
+// mixed records use).  x, y, z are the atom's position as the position
+// arrays hold it; w is F'(rho) of the site with the species in its two low
+// mantissa bits (a change of at most 3 ulp), written by the density pass.
+// A vacancy sits 10^6 A out (finite, beyond every cutoff); periodic images
+// carry their own position and the site's w.
+struct Rec64 {
+  double2* xy;
+  double2* zw;
+};
+constexpr double kVacantPos64 = 1.0e6;
+__device__ __forceinline__ double packW64(double df, int sp) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(df));
+  return __longlong_as_double(static_cast<long long>((b & ~3ull) | static_cast<unsigned long long>(sp)));
+}
+__device__ __forceinline__ int recSpecies64(double w) {
+  return static_cast<int>(static_cast<unsigned long long>(__double_as_longlong(w)) & 3ull);
+}
+// The record of slot q for an atom of type t at (x0, x1, x2), F' not yet known.
+__device__ __forceinline__ void putRec64(const Rec64& r, int64_t q, int8_t t, double x0, double x1, double x2) {
+  if (isVacant(t)) {
+    r.xy[q] = make_double2(kVacantPos64, kVacantPos64);
+    r.zw[q] = make_double2(kVacantPos64, 0.0);
+  } else {
+    r.xy[q] = make_double2(x0, x1);
+    r.zw[q] = make_double2(x2, packW64(0.0, species(t)));
+  }
+}
+
 // |v|^2 with each product and sum rounded separately; non-negative doubles
 // order like their bit patterns, so an integer maximum finds the largest.
 __device__ __forceinline__ unsigned long long speed2Bits(double a, double b, double c) {
