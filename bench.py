@@ -8,7 +8,10 @@ box centre gets 5 keV along [1 3 5] and the cascade runs with an adaptive step
 dt = min(1 fs, 0.01 A / max|v|).  A step is one complete velocity-Verlet step
 of the resident state (rho/F' pass, force pass, step length, half kicks and
 drift, ownership decisions with inter atoms, periodic halo), state resident
-in HBM; the timed steps are the first steps of the cascade.
+in HBM.  The timed steps run in the developed cascade (after 0.3 ps of
+untimed FP32-mixed steps: a few hundred interstitials, thousands of atoms a
+step taken off their sites and put back); the first steps after the recoil
+are timed as well, as a side key.
 
 Prints one JSON line: the FP32-mixed rate is `value`; the plain FP64 rate
 from the same equilibrated state, the mixed-vs-FP64 force error, the
@@ -113,16 +116,24 @@ def run_developed(dev, state, mode, steps, warmup):
     th0 = dev.thermo()
     e_start = total_energy(th0, natoms)
     t0_sim, _ = dev.sim_time()
+    dc0 = dev.defect_counters()
     dev.set_profiling(True)
     t0 = time.perf_counter()
     dev.step(steps)  # returns after the device has finished
     secs = time.perf_counter() - t0
     tim = dev.last_timings()
     dev.set_profiling(False)
+    dc1 = dev.defect_counters()
     t_sim, dt_last = dev.sim_time()
     th = dev.thermo()
     return secs, tim, {"t_window_ps": t_sim - t0_sim, "dt_last_ps": dt_last, "n_inter_start": th0["n_inter"],
                        "n_inter": th["n_inter"], "n_vacancy": th["n_vacancy"], "max_speed": th["max_speed"],
+                       # per timed step: lattice atoms taken off their sites (moved beyond 0.2a), atoms
+                       # put on a vacant site, and of those the ones farther than 0.2a from it (whose
+                       # neighbourhoods visit the full neighbour list for a step)
+                       "staged_per_step": (dc1["staged"] - dc0["staged"]) / steps,
+                       "refilled_per_step": (dc1["refilled"] - dc0["refilled"]) / steps,
+                       "refilled_far_per_step": (dc1["refilled_far"] - dc0["refilled_far"]) / steps,
                        "e_total_timed_start_eV_per_atom": e_start,
                        "e_total_timed_end_eV_per_atom": total_energy(th, natoms)}
 
