@@ -250,12 +250,13 @@ int misa_autotune(misa_handle* h, int64_t steps, int32_t* best_ty, double* best_
 /* Set / read the launch shape directly (y-tile rows, gathers in flight). */
 int misa_set_launch_shape(misa_handle* h, int32_t ty, int32_t gathers_rho, int32_t gathers_force);
 int misa_get_launch_shape(const misa_handle* h, int32_t* ty, int32_t* gathers_rho, int32_t* gathers_force);
-/* Mixed-precision force pass in persistent blocks that hold the pruned
- * offsets and the majority species' spline knots in LDS: MISA_LDS_AUTO
- * (default; boxes large enough that every block walks several virtual
- * blocks), MISA_LDS_ON, MISA_LDS_OFF.  No counterpart in the reference (a
- * tuning switch of this build's kernels); the pairs, their order and their
- * rounding are the same either way, so results are bit-identical. */
+/* Force passes (mixed and FP64) in persistent blocks that hold the majority
+ * species' spline knots in LDS (the mixed pass also the pruned offsets):
+ * MISA_LDS_AUTO (default; boxes large enough that every block walks several
+ * virtual blocks), MISA_LDS_ON, MISA_LDS_OFF.  No counterpart in the
+ * reference (a tuning switch of this build's kernels); the pairs, their
+ * order and their rounding are the same either way, so results are
+ * bit-identical. */
 #define MISA_LDS_AUTO 0
 #define MISA_LDS_ON 1
 #define MISA_LDS_OFF 2
@@ -277,13 +278,16 @@ int misa_xcd_slabs(const misa_handle* h, int32_t zlo[9]);
  * slab leaves its XCD idle) and turn balancing off. */
 int misa_set_xcd_slabs(misa_handle* h, const int32_t zlo[9]);
 /* The force pass the last force evaluation launched (-1 before any):
- * kForce (FP64), kForceM (mixed, spline segments from global memory) or
+ * kForce (FP64), kForceM (mixed, spline segments from global memory),
  * kForceL (mixed, persistent blocks with the majority species' segments in
- * LDS; the library falls back to kForceM when the device refuses its LDS
- * size). */
+ * LDS) or kForceDL (FP64, persistent blocks with the majority species'
+ * segments in LDS).  The LDS passes run where MISA_LDS_* picks them; the
+ * library falls back to kForceM / kForce when the device refuses their LDS
+ * size. */
 #define MISA_FORCE_PASS_FP64 0
 #define MISA_FORCE_PASS_MIXED 1
 #define MISA_FORCE_PASS_MIXED_LDS 2
+#define MISA_FORCE_PASS_FP64_LDS 3
 int misa_force_pass_used(const misa_handle* h, int32_t* kind);
 
 /* ---- host mirror of libcomm's domain at one rank ----

@@ -334,18 +334,23 @@ __device__ __forceinline__ void verletTail(const Geom& G, int64_t s, int p, int 
 //   MODE 1: v += c f; v += c f; x' = x + dt v; decide  (secondstep + next firststep)
 //   MODE 2: f stored only                             (first force evaluation)
 // ---------------------------------------------------------------------------
-template <typename T, int ACC, int MODE, int U>
-__global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T> tb, CXPtr x, XPtr xn,
-                                                 const int8_t* __restrict__ type, const T* __restrict__ df,
-                                                 const uint64_t* __restrict__ wmask, XPtr v, XPtr f,
-                                                 const uint64_t* __restrict__ id, VerletParams vp, StageArgs st,
-                                                 Ctl* ctl, InterView iv, Rec64 r64, Rec64 rn64) {
-  unsigned long long vb = 0;  // |v|^2 bits after the secondstep (MODE 0)
-  unsigned long long db = 0;  // |dx|^2 bits of the drift (MODE 1)
-  [&] {
+// One site of the FP64 force pass: thread tl of block b (also a virtual
+// block of the persistent launch).  With LDS, knD holds the segments of the
+// Fe-Fe pairs from knot mlo on, 7 doubles a knot: the r*phi value cubic
+// c3..c6 and the Fe density cubic c3..c5 (its c6 enters no force); a pair
+// of Fe atoms at such a knot reads them there, every other pair from the
+// tables in global memory.  Both give the same values, so the same bits.
+template <typename T, int ACC, int MODE, int U, bool LDS>
+__device__ __forceinline__ void forceSiteD(const Geom& G, const LatOffsets& O, const Tables<T>& tb, CXPtr x, XPtr xn,
+                                           const int8_t* __restrict__ type, const T* __restrict__ df,
+                                           const uint64_t* __restrict__ wmask, XPtr v, XPtr f,
+                                           const uint64_t* __restrict__ id, const VerletParams& vp,
+                                           const StageArgs& st, Ctl* ctl, const InterView& iv, const Rec64& r64,
+                                           const Rec64& rn64, const T* knD, int mlo, uint32_t b, uint32_t tl,
+                                           unsigned long long& vb, unsigned long long& db) {
   int64_t s;
   int p, cx, cy, cz;
-  if (!siteOfThread(G, s, p, cx, cy, cz)) return;
+  if (!siteOfBlock(G, b, tl, s, p, cx, cy, cz)) return;
   const int8_t ti = type[s];
   const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
   if (isVacant(ti)) {  // vacancy: keeps its stale position (atom.cpp:43-47)
@@ -405,15 +410,25 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
         ri[u] = rsqT(q2);
         const int m = knotOf(q2 * ri[u], tb.rdr, tb.nr, false, pp[u]);
         const int ej = recSpecies64(wj[u]);
-        const uint32_t off = (pairBase + static_cast<uint32_t>(ej * tb.nr + m)) *
-                             static_cast<uint32_t>(kPairRec * sizeof(T));
-        ldRec<T, kPairRec>(rtab, off, rec[u]);
-        if (ej != ei) {  // rho_ei differs from rho_ej only for unlike pairs
-          const uint32_t offi = static_cast<uint32_t>(ei * tb.nr + m) * static_cast<uint32_t>(kRhoRec * sizeof(T));
-          ldRec<T, kRhoRec>(rtabR, offi, reci[u]);
-        } else {
+        if (LDS && (ei | ej) == 0 && m >= mlo) {
+          const T* k = knD + 7 * (m - mlo);
+#pragma unroll
+          for (int q = 0; q < 7; q++) rec[u][q] = k[q];
+          rec[u][7] = T(0);
 #pragma unroll
           for (int q = 0; q < kRhoRec; q++) reci[u][q] = rec[u][4 + q];
+        } else {
+          const uint32_t off = (pairBase + static_cast<uint32_t>(ej * tb.nr + m)) *
+                               static_cast<uint32_t>(kPairRec * sizeof(T));
+          ldRec<T, kPairRec>(rtab, off, rec[u]);
+          if (ej != ei) {  // rho_ei differs from rho_ej only for unlike pairs
+            const uint32_t offi =
+                static_cast<uint32_t>(ei * tb.nr + m) * static_cast<uint32_t>(kRhoRec * sizeof(T));
+            ldRec<T, kRhoRec>(rtabR, offi, reci[u]);
+          } else {
+#pragma unroll
+            for (int q = 0; q < kRhoRec; q++) reci[u][q] = rec[u][4 + q];
+          }
         }
         dfj[u] = static_cast<T>(wj[u]);
       }
@@ -439,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
     }
   };
   // the offsets the density pass found in range for some lane of this wavefront
-  const uint32_t wv = waveId();
+  const uint32_t wv = b * (kBlock / 64) + __builtin_amdgcn_readfirstlane(tl >> 6);
   forMaskGroups<U>(uniform64(wmask[2 * wv]), uniform64(wmask[2 * wv + 1]), [&](const int* ks, int nv) {
     if (nv == U) group(std::integral_constant<int, U>{}, ks);
     else group(std::integral_constant<int, 1>{}, ks);
@@ -465,7 +480,46 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
   sum3Value<ACC>(acc, f0, f1, f2);
   checkFinite(ctl, f0, f1, f2);
   verletTail<MODE>(G, s, p, cx, cy, cz, ti, ei, xi0, xi1, xi2, f0, f1, f2, xn, v, f, id, vp, st, ctl, rn64, vb, db);
-  }();
+}
+
+template <typename T, int ACC, int MODE, int U>
+__global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T> tb, CXPtr x, XPtr xn,
+                                                 const int8_t* __restrict__ type, const T* __restrict__ df,
+                                                 const uint64_t* __restrict__ wmask, XPtr v, XPtr f,
+                                                 const uint64_t* __restrict__ id, VerletParams vp, StageArgs st,
+                                                 Ctl* ctl, InterView iv, Rec64 r64, Rec64 rn64) {
+  unsigned long long vb = 0;  // |v|^2 bits after the secondstep (MODE 0)
+  unsigned long long db = 0;  // |dx|^2 bits of the drift (MODE 1)
+  forceSiteD<T, ACC, MODE, U, false>(G, O, tb, x, xn, type, df, wmask, v, f, id, vp, st, ctl, iv, r64, rn64, nullptr,
+                                     0, blockIdx.x, threadIdx.x, vb, db);
+  if (MODE == 0 && vp.vmax) waveMaxTo(vb, vp.vmax);
+  if (MODE == 1) waveMaxTo(db, &ctl->dispmax2);
+}
+
+// The FP64 force pass in persistent blocks, one per compute unit, with the
+// Fe-Fe segments from knot mlo on in LDS (forceSiteD): per site the same
+// pairs, in the same order and with the same values, as kForce.
+template <int MODE, int U>
+__global__ __launch_bounds__(kBlockP) void kForceDL(Geom G, LatOffsets O, Tables<double> tb, CXPtr x, XPtr xn,
+                                                    const int8_t* __restrict__ type, const double* __restrict__ df,
+                                                    const uint64_t* __restrict__ wmask, XPtr v, XPtr f,
+                                                    const uint64_t* __restrict__ id, VerletParams vp, StageArgs st,
+                                                    Ctl* ctl, InterView iv, Rec64 r64, Rec64 rn64, int mlo, int nkn) {
+  extern __shared__ double knD[];
+  const double* z = tb.pairF + static_cast<size_t>(mlo) * kPairRec;  // row (Fe, Fe) is the first
+  for (int i = threadIdx.x; i < 7 * nkn; i += kBlockP) {
+    const int k = i / 7, q = i - 7 * k;
+    knD[i] = z[static_cast<size_t>(k) * kPairRec + q];
+  }
+  __syncthreads();
+  unsigned long long vb = 0, db = 0;
+  forVirtualBlocks(G, [&](uint32_t b, uint32_t tl) {
+    unsigned long long vbs = 0, dbs = 0;
+    forceSiteD<double, 0, MODE, U, true>(G, O, tb, x, xn, type, df, wmask, v, f, id, vp, st, ctl, iv, r64, rn64, knD,
+                                         mlo, b, tl, vbs, dbs);
+    vb = vbs > vb ? vbs : vb;
+    db = dbs > db ? dbs : db;
+  });
   if (MODE == 0 && vp.vmax) waveMaxTo(vb, vp.vmax);
   if (MODE == 1) waveMaxTo(db, &ctl->dispmax2);
 }
@@ -800,6 +854,17 @@ LdsKnots ldsKnotsForce(int nr) {
   k.bytes = static_cast<size_t>(32) * static_cast<size_t>(k.nkn);
   return k;
 }
+// The FP64 pass's LDS knots: 7 doubles (56 bytes) a knot, from knot mlo to
+// the last, in one block per compute unit (the kernel has no static LDS;
+// 64 bytes are kept).
+constexpr int kForceLdsBytesD = 160 * 1024 - 64;
+LdsKnots ldsKnotsForceD(int nr) {
+  LdsKnots k;
+  k.nkn = std::min(nr, kForceLdsBytesD / 56);
+  k.mlo = nr - k.nkn;
+  k.bytes = static_cast<size_t>(56) * static_cast<size_t>(k.nkn);
+  return k;
+}
 bool setLdsBytes(const void* fn, int bytes) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess;
 }
@@ -857,13 +922,25 @@ int launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t6
   const XPtr xn = xp(s.x[1 - s.cur]);
   const StageArgs sa = stageOf(s);
   const InterView iv = interView(s, mixed);
-  const bool lds = mixed && useLds(s);
+  const bool lds = useLds(s);
   const LdsKnots lk = ldsKnotsForce(t32.nr);
-#define MISA_FORCE(M)                                                                                       \
-  kForce<double, 0, M, kUDouble><<<grid, kBlock, 0, st>>>(s.G, o, t64, x, xn, s.type, s.df64, s.wmask, xp(s.v), \
-                                                         xp(s.f), s.id, vp, sa, s.ctl, iv,                   \
-                                                         Rec64{s.pxy[s.cur], s.pzw[s.cur]},                  \
-                                                         Rec64{s.pxy[1 - s.cur], s.pzw[1 - s.cur]})
+  const LdsKnots lkd = ldsKnotsForceD(t64.nr);
+#define MISA_FORCE(M)                                                                                         \
+  do {                                                                                                        \
+    static const bool ldsReady =                                                                              \
+        setLdsBytes(reinterpret_cast<const void*>(&kForceDL<M, kUDouble>), kForceLdsBytesD);                 \
+    const Rec64 r64{s.pxy[s.cur], s.pzw[s.cur]}, rn64{s.pxy[1 - s.cur], s.pzw[1 - s.cur]};                   \
+    if (lds && ldsReady) {                                                                                    \
+      kind = 3;                                                                                               \
+      kForceDL<M, kUDouble><<<persistentGrid(s), kBlockP, lkd.bytes, st>>>(                                   \
+          s.G, o, t64, x, xn, s.type, s.df64, s.wmask, xp(s.v), xp(s.f), s.id, vp, sa, s.ctl, iv, r64, rn64,   \
+          lkd.mlo, lkd.nkn);                                                                                  \
+    } else {                                                                                                  \
+      kForce<double, 0, M, kUDouble><<<grid, kBlock, 0, st>>>(s.G, o, t64, x, xn, s.type, s.df64, s.wmask,    \
+                                                             xp(s.v), xp(s.f), s.id, vp, sa, s.ctl, iv, r64, \
+                                                             rn64);                                          \
+    }                                                                                                         \
+  } while (0)
 #define MISA_FORCE_MU(A, M, U)                                                                                \
   do {                                                                                                        \
     static const bool ldsReady =                                                                              \

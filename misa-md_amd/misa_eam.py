@@ -356,7 +356,7 @@ class EamDevice:
         """Name of the force kernel the last force evaluation launched."""
         k = ctypes.c_int32()
         self._check(self.lib.misa_force_pass_used(self.h, ctypes.byref(k)), "misa_force_pass_used")
-        return {0: "kForce", 1: "kForceM", 2: "kForceL"}.get(k.value, "none")
+        return {0: "kForce", 1: "kForceM", 2: "kForceL", 3: "kForceDL"}.get(k.value, "none")
 
     def set_xcd_balance(self, on):
         """Cut the XCD z-slabs by the force pass's measured work (default on)."""

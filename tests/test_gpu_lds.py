@@ -1,12 +1,12 @@
-"""The mixed force pass in persistent blocks with the majority species'
-spline knots in LDS (needs an MI355X).
+"""The force passes (mixed and FP64) in persistent blocks with the majority
+species' spline knots in LDS (needs an MI355X).
 
-The LDS pass visits the same pairs, in the same order, with the same
-rounding as the pass that gathers every segment from global memory, so a box
-stepped with it must end bit for bit where it ends without it: in every
-mixed-precision mode, with either count of gathers in flight, through a
-recoil (unlike pairs and close pairs take the global tables), and on a fine
-grid whose first knots lie below the LDS range.
+An LDS pass visits the same pairs, in the same order, with the same values
+and rounding as the pass that gathers every segment from global memory, so a
+box stepped with it must end bit for bit where it ends without it: in every
+precision mode, with either count of gathers in flight, through a recoil
+(unlike pairs and close pairs take the global tables), and on a fine grid
+whose first knots lie below the LDS range.
 """
 import numpy as np
 import pytest
@@ -50,7 +50,7 @@ def _same(a, b, what):
     return len(ia)
 
 
-@pytest.mark.parametrize("precision", [M.PREC_MIXED_KAHAN, M.PREC_MIXED, M.PREC_MIXED_F32SUM])
+@pytest.mark.parametrize("precision", [M.PREC_MIXED_KAHAN, M.PREC_MIXED, M.PREC_MIXED_F32SUM, M.PREC_FP64])
 @pytest.mark.parametrize("gathers", [4, 2])
 def test_lds_force_pass_gives_one_trajectory(setfl_fecuni, precision, gathers):
     cells = (8, 8, 12)
@@ -74,12 +74,13 @@ def test_lds_force_pass_gives_one_trajectory(setfl_fecuni, precision, gathers):
     on.close()
 
 
-def test_lds_fine_grid_falls_back_below_the_table(tmp_path):
-    """8000 knots: the LDS holds knots from about 2.1 A on, so the closest
-    pairs of a 1200 K box take the global tables."""
+@pytest.mark.parametrize("precision", [M.PREC_MIXED_KAHAN, M.PREC_FP64])
+def test_lds_fine_grid_falls_back_below_the_table(tmp_path, precision):
+    """8000 knots: the LDS holds knots from about 2.1 A (mixed) or 3.6 A
+    (FP64) on, so the closer pairs of a 1200 K box take the global tables."""
     path = setfl_gen.write_setfl(str(tmp_path / "fine.eam.alloy"), ("Fe", "Cu", "Ni"), nr=8000, nrho=3000,
                                  rho_max=60.0)
-    off, on = _pair(path, (7, 6, 8), 43, 1200.0, M.PREC_MIXED_KAHAN)
+    off, on = _pair(path, (7, 6, 8), 43, 1200.0, precision)
     _same(off, on, "first forces")
     for dev in (off, on):
         dev.step(30)
