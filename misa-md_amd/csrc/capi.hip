@@ -50,6 +50,7 @@ struct misa_handle {
   bool forces_ready = false;
   uint64_t gen = 0;  // bumped by every change a captured step graph would bake in
   uint64_t skin_gen = ~0ull;  // gen at which the density pass's skin masks were last invalidated
+  uint64_t mask_gen = ~0ull;  // gen of the last density pass (its wave masks fit the current slab cut)
   AtomElement* aos_dev = nullptr;
   std::vector<void*> allocs;
   bool profile = false;
@@ -446,8 +447,9 @@ int misa_create(const misa_geom* g, int device, misa_handle** out) {
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess) s.ncu = ncu;
   }
   const int64_t blocks = 16 * h->bpx_cap;
-  // room for the site kernels' 7 partials per block and the inter kernels' 5
-  s.npartials = std::max<int64_t>(8 * blocks, 8 * kInterBlocks);
+  // room for the site kernels' 7 partials per block, the inter kernels' 5
+  // and the persistent energy pass's 3 per compute unit
+  s.npartials = std::max<int64_t>(std::max<int64_t>(8 * blocks, 8 * kInterBlocks), 3 * static_cast<int64_t>(s.ncu));
   if (rc == MISA_OK) rc |= devAlloc(h, &s.partials, static_cast<size_t>(s.npartials));
   if (rc == MISA_OK) rc |= allocInter(h, 65536);
   if (rc != MISA_OK) {
@@ -925,6 +927,7 @@ static void forcePass(misa_handle* h, int lattice_mode, int inter_mode, bool sto
   {
     Timer t(h, 0);
     launchRhoDf(h->s, h->lat, h->t64, h->t32, mixed, h->acc, store_rho, h->stream);
+    h->mask_gen = h->gen;
   }
   {
     Timer t(h, 2);
@@ -1189,7 +1192,11 @@ int misa_thermo_compute(misa_handle* h, misa_thermo* out) {
   const uint64_t dof = 3 * n - 3;
   out->temperature = out->mvv * kMvv2e / (dof * kBoltz);
   if (h->have_pot && h->have_offsets && h->forces_ready) {
-    const int64_t ne = launchEnergy(h->s, h->lat, h->t64, h->precision == MISA_PREC_MIXED, h->stream);
+    // the energy pass visits the skin masks' offsets only while they fit
+    // the current slab cut (a re-cut since the last density pass moves sites
+    // between wavefronts) and hold a skin
+    const bool all = !(h->s.skin > 0.0) || h->mask_gen != h->gen;
+    const int64_t ne = launchEnergy(h->s, h->lat, h->t64, all, h->stream);
     HIPCHECK(h, hipGetLastError());
     std::vector<double> q(static_cast<size_t>(3 * ne));
     HIPCHECK(h, hipMemcpyAsync(q.data(), h->s.partials, q.size() * 8, hipMemcpyDeviceToHost, h->stream));

@@ -496,6 +496,186 @@ __global__ __launch_bounds__(kBlock) void kForce(Geom G, LatOffsets O, Tables<T>
   if (MODE == 1) waveMaxTo(db, &ctl->dispmax2);
 }
 
+// Potential energy and virial of site s (FP64 tables, FP64 site records):
+// F(rho_i) into vals[0], 1/2 sum_j phi(r_ij) into vals[1] and
+// 1/2 sum_j r_ij^2 fpair_ij into vals[2], fpair as the force pass computes it
+// and F'(rho_j) from the records' w.  The offsets are those of the
+// wavefront's skin mask (within the cutoff plus the skin of every pair at the
+// last full density scan, so a superset of the pairs within the cutoff), or
+// all pruned offsets when all is set (no skin).  With LDS the Fe-Fe segments
+// come from knD as in forceSiteD.  Near a far lattice atom the rest of the
+// full list follows from x and df.
+template <bool LDS>
+__device__ __forceinline__ void energySiteD(const Geom& G, const LatOffsets& O, const FullPlaneOffsets* full,
+                                           const Tables<double>& tb, CXPtr x, const int8_t* __restrict__ type,
+                                           const double* __restrict__ rho, const double* __restrict__ df,
+                                           const uint64_t* __restrict__ smask, bool all, const Rec64& r64,
+                                           const double* knD, int mlo, uint32_t b, uint32_t tl,
+                                           double (&vals)[3]) {
+  int64_t s;
+  int p, cx, cy, cz;
+  if (!siteOfBlock(G, b, tl, s, p, cx, cy, cz)) return;
+  const int8_t ti = type[s];
+  if (isVacant(ti)) return;
+  const int ei = species(ti);
+  const double ri = rho[s];
+  const double dfi = dEmbed(tb, ei, ri);
+  vals[0] += embed(tb, ei, ri);
+  const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
+  const Rsrc rxy = rsrcBelow(r64.xy, O.lo), rzw = rsrcBelow(r64.zw, O.lo);
+  const Rsrc rtab = rsrcAt(reinterpret_cast<uintptr_t>(tb.pairF));
+  const Rsrc rtabR = rsrcAt(reinterpret_cast<uintptr_t>(tb.rhoV));
+  const uint32_t o16 = static_cast<uint32_t>(s) * 16u;
+  const uint32_t pairBase = static_cast<uint32_t>(ei * 3) * static_cast<uint32_t>(tb.nr);
+  const uint32_t* sh = O.sh[p];
+  auto group = [&](auto UC, const int* ks) {
+    constexpr int V = decltype(UC)::value;
+    double d0[V], d1[V], d2[V], wj[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      const uint32_t so = sh[ks[u]] * 16u;
+      const double2 xy = ldRec64(rxy, o16, so), zw = ldRec64(rzw, o16, so);
+      d0[u] = xi0 - xy.x;
+      d1[u] = xi1 - xy.y;
+      d2[u] = xi2 - zw.x;
+      wj[u] = zw.y;
+    }
+    bool in[V];
+    double r2[V], pp[V], rinv[V], rec[V][kPairRec], reci[V][kRhoRec];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      r2[u] = dist2D(d0[u], d1[u], d2[u]);
+      in[u] = r2[u] < G.rc2;
+      if (in[u]) {
+        rinv[u] = rsqT(r2[u]);
+        const int m = knotOf(r2[u] * rinv[u], tb.rdr, tb.nr, false, pp[u]);
+        const int ej = recSpecies64(wj[u]);
+        if (LDS && (ei | ej) == 0 && m >= mlo) {
+          const double* k = knD + 7 * (m - mlo);
+#pragma unroll
+          for (int q = 0; q < 7; q++) rec[u][q] = k[q];
+          rec[u][7] = 0.0;
+#pragma unroll
+          for (int q = 0; q < kRhoRec; q++) reci[u][q] = rec[u][4 + q];
+        } else {
+          ldRec<double, kPairRec>(rtab, (pairBase + static_cast<uint32_t>(ej * tb.nr + m)) * 64u, rec[u]);
+          if (ej != ei) {
+            ldRec<double, kRhoRec>(rtabR, static_cast<uint32_t>(ei * tb.nr + m) * 32u, reci[u]);
+          } else {
+#pragma unroll
+            for (int q = 0; q < kRhoRec; q++) reci[u][q] = rec[u][4 + q];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      if (!in[u]) continue;
+      const double q = pp[u];
+      const double* z = rec[u];
+      const double* rj = rec[u] + 4;
+      const double* rI = reci[u];
+      const double z2 = ((z[0] * q + z[1]) * q + z[2]) * q + z[3];
+      const double z2p = ((3.0 * z[0] * q + 2.0 * z[1]) * q + z[2]) * tb.rdr;
+      const double rhojp = ((3.0 * rj[0] * q + 2.0 * rj[1]) * q + rj[2]) * tb.rdr;
+      const double rhoip = ((3.0 * rI[0] * q + 2.0 * rI[1]) * q + rI[2]) * tb.rdr;
+      const double phi = z2 * rinv[u];
+      const double phip = z2p * rinv[u] - phi * rinv[u];
+      const double psip = dfi * rhojp + wj[u] * rhoip + phip;
+      vals[1] += 0.5 * phi;
+      vals[2] += 0.5 * r2[u] * (-psip * rinv[u]);
+    }
+  };
+  if (all) {
+    const int nk = O.n[p];
+    int ks[2];
+    int k = 0;
+    for (; k + 2 <= nk; k += 2) {
+      ks[0] = k;
+      ks[1] = k + 1;
+      group(std::integral_constant<int, 2>{}, ks);
+    }
+    if (k < nk) {
+      ks[0] = k;
+      group(std::integral_constant<int, 1>{}, ks);
+    }
+  } else {
+    const uint32_t wv = b * (kBlock / 64) + __builtin_amdgcn_readfirstlane(tl >> 6);
+    forMaskGroups<2>(uniform64(smask[2 * wv]), uniform64(smask[2 * wv + 1]), [&](const int* ks, int nv) {
+      if (nv == 2) group(std::integral_constant<int, 2>{}, ks);
+      else group(std::integral_constant<int, 1>{}, ks);
+    });
+  }
+  if (ti & kWide) {  // full-list offsets beyond the pruned list, near a far lattice atom
+    forWideTargets(*full, type, p, s, ti, [&](int64_t j, int8_t tj) {
+      const double e0 = xi0 - x.c[0][j], e1 = xi1 - x.c[1][j], e2 = xi2 - x.c[2][j];
+      const double q2 = e0 * e0 + e1 * e1 + e2 * e2;
+      if (q2 < G.rc2) {
+        const int ej = species(tj);
+        const double r = sqrt(q2);
+        double q;
+        const int m = knotOf(r, tb.rdr, tb.nr, false, q);
+        vals[1] += 0.5 * splineVal(tb.z2 + pairIdx(ei, ej) * tb.sR + static_cast<int64_t>(m) * kKnot, q) / r;
+        vals[2] += 0.5 * q2 * pairForce(tb, ei, ej, q2, dfi, df[j]);
+      }
+    });
+  }
+}
+
+// Sums of a block's threads (fixed order), one value each into out.
+template <int N, int THREADS>
+__device__ __forceinline__ void blockSumD(double (&vals)[N], double* out) {
+  __shared__ double red[N][THREADS / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < N; q++) {
+    double v = vals[q];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) red[q][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+      double v = 0;
+      for (int k = 0; k < THREADS / 64; k++) v += red[q][k];
+      out[q] = v;
+    }
+  }
+}
+
+// Potential energy and virial, one block's partial sums each: energySiteD
+// over the sites of a siteGrid launch, or in persistent blocks with the
+// Fe-Fe segments in LDS.
+__global__ __launch_bounds__(kBlock) void kEnergyD(Geom G, LatOffsets O, const FullPlaneOffsets* __restrict__ full,
+                                                   Tables<double> tb, CXPtr x, const int8_t* __restrict__ type,
+                                                   const double* __restrict__ rho, const double* __restrict__ df,
+                                                   const uint64_t* __restrict__ smask, int all, Rec64 r64,
+                                                   double* partials) {
+  double vals[3] = {0, 0, 0};
+  energySiteD<false>(G, O, full, tb, x, type, rho, df, smask, all != 0, r64, nullptr, 0, blockIdx.x, threadIdx.x,
+                     vals);
+  blockSumD<3, kBlock>(vals, partials + 3 * blockIdx.x);
+}
+__global__ __launch_bounds__(kBlockP) void kEnergyDL(Geom G, LatOffsets O, const FullPlaneOffsets* __restrict__ full,
+                                                     Tables<double> tb, CXPtr x, const int8_t* __restrict__ type,
+                                                     const double* __restrict__ rho, const double* __restrict__ df,
+                                                     const uint64_t* __restrict__ smask, int all, Rec64 r64,
+                                                     double* partials, int mlo, int nkn) {
+  extern __shared__ double knD[];
+  const double* z = tb.pairF + static_cast<size_t>(mlo) * kPairRec;  // row (Fe, Fe) is the first
+  for (int i = threadIdx.x; i < 7 * nkn; i += kBlockP) {
+    const int k = i / 7, q = i - 7 * k;
+    knD[i] = z[static_cast<size_t>(k) * kPairRec + q];
+  }
+  __syncthreads();
+  double vals[3] = {0, 0, 0};
+  forVirtualBlocks(G, [&](uint32_t b, uint32_t tl) {
+    energySiteD<true>(G, O, full, tb, x, type, rho, df, smask, all != 0, r64, knD, mlo, b, tl, vals);
+  });
+  blockSumD<3, kBlockP>(vals, partials + 3 * blockIdx.x);
+}
+
 // The FP64 force pass in persistent blocks, one per compute unit, with the
 // Fe-Fe segments from knot mlo on in LDS (forceSiteD): per site the same
 // pairs, in the same order and with the same values, as kForce.
@@ -865,6 +1045,16 @@ LdsKnots ldsKnotsForceD(int nr) {
   k.bytes = static_cast<size_t>(56) * static_cast<size_t>(k.nkn);
   return k;
 }
+// The energy pass's LDS knots: as the FP64 force pass's, beside its static
+// LDS (the block sums; 512 bytes are kept).
+constexpr int kEnergyLdsBytes = 160 * 1024 - 512;
+LdsKnots ldsKnotsEnergy(int nr) {
+  LdsKnots k;
+  k.nkn = std::min(nr, kEnergyLdsBytes / 56);
+  k.mlo = nr - k.nkn;
+  k.bytes = static_cast<size_t>(56) * static_cast<size_t>(k.nkn);
+  return k;
+}
 bool setLdsBytes(const void* fn, int bytes) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess;
 }
@@ -983,6 +1173,24 @@ int launchForce(const DevState& s, const LatOffsets& o, const Tables<double>& t6
 #undef MISA_FORCE_M
 #undef MISA_FORCE_MU
   return kind;
+}
+
+int64_t launchEnergyPass(const DevState& s, const LatOffsets& o, const Tables<double>& t64, bool every,
+                         hipStream_t st) {
+  static const bool ldsReady = setLdsBytes(reinterpret_cast<const void*>(&kEnergyDL), kEnergyLdsBytes);
+  const Rec64 r64{s.pxy[s.cur], s.pzw[s.cur]};
+  const int all = every ? 1 : 0;
+  if (useLds(s) && ldsReady && 3 * static_cast<int64_t>(persistentGrid(s)) <= s.npartials) {
+    const LdsKnots lk = ldsKnotsEnergy(t64.nr);
+    const unsigned grid = persistentGrid(s);
+    kEnergyDL<<<grid, kBlockP, lk.bytes, st>>>(s.G, o, s.in.full, t64, cxp(s.x[s.cur]), s.type, s.rho, s.df64,
+                                               s.smask, all, r64, s.partials, lk.mlo, lk.nkn);
+    return grid;
+  }
+  const unsigned grid = siteGrid(s.G);  // 3 partials per block fit: the buffer holds 7 per site block
+  kEnergyD<<<grid, kBlock, 0, st>>>(s.G, o, s.in.full, t64, cxp(s.x[s.cur]), s.type, s.rho, s.df64, s.smask, all,
+                                    r64, s.partials);
+  return grid;
 }
 
 void launchRhoDfBatch(const BatchBox* boxes, const uint32_t* rhoStart, int nbox, unsigned blocks, int acc,

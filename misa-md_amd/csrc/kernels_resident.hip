@@ -257,81 +257,18 @@ __global__ __launch_bounds__(kBlock) void kThermo(Geom G, CXPtr v, CXPtr f, cons
 }
 
 // F'(rho) with the FP64 tables for every occupied slot, ghosts included,
-// from the stored densities: what the FP64 density pass leaves in df, for
-// the energy after a mixed-precision step.
+// from the stored densities, over every slot: what the FP64 density pass
+// leaves in df and in the w of the FP64 site records, for the energy pass.
 __global__ __launch_bounds__(kBlock) void kDfOfRho(Geom G, Tables<double> tb, const int8_t* __restrict__ type,
-                                                   const double* __restrict__ rho, double* __restrict__ df) {
+                                                   const double* __restrict__ rho, double* __restrict__ df,
+                                                   double2* __restrict__ zw) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (t >= 2 * G.plane) return;
   const int8_t ti = type[t];
-  if (!isVacant(ti)) df[t] = dEmbed(tb, species(ti), rho[t]);
-}
-
-// Potential energy and virial with FP64 tables over the full list:
-// per site F(rho_i) + 1/2 sum_j phi(r_ij) and 1/2 sum_j r_ij^2 fpair_ij, with
-// F'(rho_j) read from df (dEmbed of the stored densities).  The pruned list
-// goes in groups of four whose loads are all issued before the first pair
-// term; near a far lattice atom the rest of the full list follows one by
-// one.  The terms are added in list order.
-__global__ __launch_bounds__(kBlock) void kEnergy(Geom G, LatOffsets O, const FullPlaneOffsets* __restrict__ full,
-                                                  Tables<double> tb, CXPtr x, const int8_t* __restrict__ type,
-                                                  const double* __restrict__ rho, const double* __restrict__ df,
-                                                  double* partials) {
-  int64_t s;
-  int p, cx, cy, cz;
-  double vals[3] = {0, 0, 0};
-  if (siteOfThread(G, s, p, cx, cy, cz)) {
-    const int8_t ti = type[s];
-    if (!isVacant(ti)) {
-      const int ei = species(ti);
-      const double ri = rho[s];
-      const double dfi = dEmbed(tb, ei, ri);
-      vals[0] = embed(tb, ei, ri);
-      const double xi0 = x.c[0][s], xi1 = x.c[1][s], xi2 = x.c[2][s];
-      auto pair = [&](int8_t tj, double xj0, double xj1, double xj2, double dfj) {
-        const double d0 = xi0 - xj0, d1 = xi1 - xj1, d2 = xi2 - xj2;
-        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        if (!isVacant(tj) && r2 < G.rc2) {
-          const int ej = species(tj);
-          const double r = sqrt(r2);
-          double q;
-          const int m = knotOf(r, tb.rdr, tb.nr, false, q);
-          vals[1] += 0.5 * splineVal(tb.z2 + pairIdx(ei, ej) * tb.sR + static_cast<int64_t>(m) * kKnot, q) / r;
-          vals[2] += 0.5 * r2 * pairForce(tb, ei, ej, r2, dfi, dfj);
-        }
-      };
-      constexpr int V = 4;
-      const int np = O.n[p];
-      int k = 0;
-      for (; k + V <= np; k += V) {
-        int8_t tj[V];
-        double xj0[V], xj1[V], xj2[V], dfj[V];
-#pragma unroll
-        for (int u = 0; u < V; ++u) {
-          const int64_t j = s + O.off[p][k + u];
-          tj[u] = type[j];
-          xj0[u] = x.c[0][j];
-          xj1[u] = x.c[1][j];
-          xj2[u] = x.c[2][j];
-          dfj[u] = df[j];
-        }
-#pragma unroll
-        for (int u = 0; u < V; ++u) pair(tj[u], xj0[u], xj1[u], xj2[u], dfj[u]);
-      }
-      for (; k < np; ++k) {
-        const int64_t j = s + O.off[p][k];
-        pair(type[j], x.c[0][j], x.c[1][j], x.c[2][j], df[j]);
-      }
-      // near a far lattice atom, the rest of the full list
-      const int nw = (ti & kWide) ? full->nw[p] : 0;
-      for (int w = 0; w < nw; ++w) {
-        const int64_t j = s + full->woff[p][w];
-        const int8_t tj = type[j];
-        if (wideTarget(ti, tj)) pair(tj, x.c[0][j], x.c[1][j], x.c[2][j], df[j]);
-      }
-    }
-  }
-  blockSum<3>(vals, partials + 3 * blockIdx.x);
+  if (isVacant(ti)) return;
+  const double d = dEmbed(tb, species(ti), rho[t]);
+  df[t] = d;
+  zw[t].y = packW64(d, species(ti));
 }
 
 __global__ __launch_bounds__(kBlock) void kScaleV(Geom G, XPtr v, double factor) {
@@ -389,15 +326,15 @@ int64_t launchThermo(const DevState& s, hipStream_t st) {
   return grid;
 }
 
-int64_t launchEnergy(const DevState& s, const LatOffsets& o, const Tables<double>& t64, bool mixed,
-                     hipStream_t st) {
-  if (mixed) {  // the FP64 density pass has not left F' in df
-    const unsigned g = static_cast<unsigned>((2 * s.G.plane + kBlock - 1) / kBlock);
-    kDfOfRho<<<g, kBlock, 0, st>>>(s.G, t64, s.type, s.rho, s.df64);
-  }
-  const unsigned grid = siteGrid(s.G);
-  kEnergy<<<grid, kBlock, 0, st>>>(s.G, o, s.in.full, t64, cxp(s.x[s.cur]), s.type, s.rho, s.df64, s.partials);
-  return grid;
+int64_t launchEnergy(const DevState& s, const LatOffsets& o, const Tables<double>& t64, bool all, hipStream_t st) {
+  // the FP64 site records of the current positions with F' of the stored
+  // densities in w, whichever precision and step left the state (a mixed
+  // step keeps no FP64 records; in FP64 the values come out the same)
+  const unsigned g = static_cast<unsigned>((2 * s.G.plane + kBlock - 1) / kBlock);
+  const Rec64 r64{s.pxy[s.cur], s.pzw[s.cur]};
+  kPackRec64<<<g, kBlock, 0, st>>>(s.G, cxp(s.x[s.cur]), s.type, r64);
+  kDfOfRho<<<g, kBlock, 0, st>>>(s.G, t64, s.type, s.rho, s.df64, r64.zw);
+  return launchEnergyPass(s, o, t64, all, st);
 }
 
 void launchScaleV(const DevState& s, double factor, hipStream_t st) {

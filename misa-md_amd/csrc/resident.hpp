@@ -132,8 +132,13 @@ void launchPackRec(const DevState& s, int buf, bool fp64, hipStream_t st);
 // Partial sums per block: [mvv, n_lat, n_vac, fx, fy, fz, vmax2] (7 per block)
 int64_t launchThermo(const DevState& s, hipStream_t st);
 // Partial sums per block: [e_embed, e_pair, virial] (3 per block)
-int64_t launchEnergy(const DevState& s, const LatOffsets& o, const Tables<double>& t64, bool mixed,
-                     hipStream_t st);
+// all: visit every pruned offset (else the wavefronts' skin masks, which
+// must fit the current slab cut and positions).
+int64_t launchEnergy(const DevState& s, const LatOffsets& o, const Tables<double>& t64, bool all, hipStream_t st);
+// The energy pass proper (kernels_hot.hip), on FP64 records and F' that
+// launchEnergy has just packed.
+int64_t launchEnergyPass(const DevState& s, const LatOffsets& o, const Tables<double>& t64, bool all,
+                         hipStream_t st);
 void launchScaleV(const DevState& s, double factor, hipStream_t st);
 // Adaptive step: max |v|^2 over lattice and inter atoms into Ctl::vmax2, then
 // Ctl::dt = min(dt_max, max_disp / max|v|) and Ctl::t_sim += Ctl::dt; with

@@ -32,6 +32,13 @@ def _same(a, b, what):
     return len(ia)
 
 
+def _same_energy(a, b, what):
+    """The same terms in another order: FP64 sums agree to rounding."""
+    ta, tb = a.thermo(), b.thermo()
+    for k in ("e_potential", "virial"):
+        assert tb[k] == pytest.approx(ta[k], rel=1e-12), "%s: %s" % (what, k)
+
+
 def _even(n2):
     return [c * n2 // 8 for c in range(9)]
 
@@ -72,6 +79,7 @@ def test_any_cut_gives_one_trajectory(setfl_fecuni):
         dev.collision_step((8, 8, 20, 0), (1.0, 3.0, 5.0), 1500.0)
         dev.step(25)  # the fixed step: the force pass fused with the next half step
     assert _same(even, cut, "after the recoil") > 0
+    _same_energy(even, cut, "after the recoil")
     for dev in (even, cut):
         dev.set_adaptive_dt(0.001, 0.01)
         dev.step(20)
@@ -103,5 +111,8 @@ def test_measured_cut_thins_the_cascade_slab(setfl_fecuni):
     assert thin, "no slab was cut thinner than the even eighth"
     assert all(zlo[c + 1] > 40 for c in thin), "slabs away from the cascade were thinned"
     assert _same(even, bal, "after the cascade") > 0
+    # right after the step call that re-cut the slabs, the energy pass must
+    # not take the last density pass's wave masks for the new cut's waves
+    _same_energy(even, bal, "after the cascade")
     for dev in (even, bal):
         dev.close()

@@ -59,6 +59,11 @@ def test_lds_force_pass_gives_one_trajectory(setfl_fecuni, precision, gathers):
     for dev in (off, on):
         dev.set_launch_shape(4, 4, gathers)
     _same(off, on, "first forces")
+    # the energy pass also runs in persistent blocks with LDS tables: the
+    # same terms, summed in another order
+    eo, en = off.thermo(), on.thermo()
+    assert en["e_potential"] == pytest.approx(eo["e_potential"], rel=1e-13)
+    assert en["virial"] == pytest.approx(eo["virial"], rel=1e-12)
     for dev in (off, on):
         dev.step(10)
         dev.collision_step((4, 4, 6, 0), (1.0, 3.0, 5.0), 600.0)
