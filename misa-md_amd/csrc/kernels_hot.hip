@@ -334,6 +334,34 @@ __device__ __forceinline__ void verletTail(const Geom& G, int64_t s, int p, int 
 //   MODE 1: v += c f; v += c f; x' = x + dt v; decide  (secondstep + next firststep)
 //   MODE 2: f stored only                             (first force evaluation)
 // ---------------------------------------------------------------------------
+// The FP64 segments of a pair at knot m: rec = the r*phi value cubic c3..c6
+// and the neighbour's density cubic c3..c6, reci = the centre's density
+// cubic (rec's for a like pair).  With LDS a pair of Fe atoms at a knot from
+// mlo on reads them from knD (7 doubles a knot, the density's c6 left out:
+// no force or energy term uses it), every other pair from the tables in
+// global memory; the values are the same.
+template <bool LDS>
+__device__ __forceinline__ void pairSegD(Rsrc rtab, Rsrc rtabR, const double* knD, int mlo, uint32_t pairBase,
+                                         int nr, int ei, int ej, int m, double (&rec)[kPairRec],
+                                         double (&reci)[kRhoRec]) {
+  if (LDS && (ei | ej) == 0 && m >= mlo) {
+    const double* k = knD + 7 * (m - mlo);
+#pragma unroll
+    for (int q = 0; q < 7; q++) rec[q] = k[q];
+    rec[7] = 0.0;
+#pragma unroll
+    for (int q = 0; q < kRhoRec; q++) reci[q] = rec[4 + q];
+    return;
+  }
+  ldRec<double, kPairRec>(rtab, (pairBase + static_cast<uint32_t>(ej * nr + m)) * (kPairRec * 8u), rec);
+  if (ej != ei) {  // rho_ei differs from rho_ej only for unlike pairs
+    ldRec<double, kRhoRec>(rtabR, static_cast<uint32_t>(ei * nr + m) * (kRhoRec * 8u), reci);
+  } else {
+#pragma unroll
+    for (int q = 0; q < kRhoRec; q++) reci[q] = rec[4 + q];
+  }
+}
+
 // One site of the FP64 force pass: thread tl of block b (also a virtual
 // block of the persistent launch).  With LDS, knD holds the segments of the
 // Fe-Fe pairs from knot mlo on, 7 doubles a knot: the r*phi value cubic
@@ -409,27 +437,7 @@ __device__ __forceinline__ void forceSiteD(const Geom& G, const LatOffsets& O, c
         const T q2 = static_cast<T>(r2);
         ri[u] = rsqT(q2);
         const int m = knotOf(q2 * ri[u], tb.rdr, tb.nr, false, pp[u]);
-        const int ej = recSpecies64(wj[u]);
-        if (LDS && (ei | ej) == 0 && m >= mlo) {
-          const T* k = knD + 7 * (m - mlo);
-#pragma unroll
-          for (int q = 0; q < 7; q++) rec[u][q] = k[q];
-          rec[u][7] = T(0);
-#pragma unroll
-          for (int q = 0; q < kRhoRec; q++) reci[u][q] = rec[u][4 + q];
-        } else {
-          const uint32_t off = (pairBase + static_cast<uint32_t>(ej * tb.nr + m)) *
-                               static_cast<uint32_t>(kPairRec * sizeof(T));
-          ldRec<T, kPairRec>(rtab, off, rec[u]);
-          if (ej != ei) {  // rho_ei differs from rho_ej only for unlike pairs
-            const uint32_t offi =
-                static_cast<uint32_t>(ei * tb.nr + m) * static_cast<uint32_t>(kRhoRec * sizeof(T));
-            ldRec<T, kRhoRec>(rtabR, offi, reci[u]);
-          } else {
-#pragma unroll
-            for (int q = 0; q < kRhoRec; q++) reci[u][q] = rec[u][4 + q];
-          }
-        }
+        pairSegD<LDS>(rtab, rtabR, knD, mlo, pairBase, tb.nr, ei, recSpecies64(wj[u]), m, rec[u], reci[u]);
         dfj[u] = static_cast<T>(wj[u]);
       }
     }
@@ -549,23 +557,7 @@ __device__ __forceinline__ void energySiteD(const Geom& G, const LatOffsets& O, 
       if (in[u]) {
         rinv[u] = rsqT(r2[u]);
         const int m = knotOf(r2[u] * rinv[u], tb.rdr, tb.nr, false, pp[u]);
-        const int ej = recSpecies64(wj[u]);
-        if (LDS && (ei | ej) == 0 && m >= mlo) {
-          const double* k = knD + 7 * (m - mlo);
-#pragma unroll
-          for (int q = 0; q < 7; q++) rec[u][q] = k[q];
-          rec[u][7] = 0.0;
-#pragma unroll
-          for (int q = 0; q < kRhoRec; q++) reci[u][q] = rec[u][4 + q];
-        } else {
-          ldRec<double, kPairRec>(rtab, (pairBase + static_cast<uint32_t>(ej * tb.nr + m)) * 64u, rec[u]);
-          if (ej != ei) {
-            ldRec<double, kRhoRec>(rtabR, static_cast<uint32_t>(ei * tb.nr + m) * 32u, reci[u]);
-          } else {
-#pragma unroll
-            for (int q = 0; q < kRhoRec; q++) reci[u][q] = rec[u][4 + q];
-          }
-        }
+        pairSegD<LDS>(rtab, rtabR, knD, mlo, pairBase, tb.nr, ei, recSpecies64(wj[u]), m, rec[u], reci[u]);
       }
     }
 #pragma unroll
